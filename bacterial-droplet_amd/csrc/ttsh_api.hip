@@ -1,0 +1,798 @@
+// ttsh_api.hip -- C ABI of libttsh3d: device memory, step sequencing, HIP
+// graphs and batches of droplets.
+//
+// An Engine owns a set of droplet slots on one device and one stream.  A step
+// of the engine is six launches, each covering every active droplet through a
+// work table (see ttsh_dev.h):
+//   K_rhs -> K_divx -> K_ffty(fwd) -> K_fftz -> K_ffty(inv) -> K_corrx
+// which stands in for the reference's EEuler (TTSH3D.cu:347-359: 107 kernel
+// launches and 4 cuFFT transforms per step).
+//
+// ttsh_sim is an engine with one slot, stepped with plain launches.  A
+// ttsh_batch may capture K steps in a HIP graph and replay it; the kernels
+// then run with a fixed grid and read the work count from device memory, so
+// admitting or retiring droplets rewrites tables without re-capturing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ttsh3d.h"
+#include "ttsh_dev.h"
+#include "ttsh_internal.h"
+#include "ttsh_kernels.h"
+
+namespace ttsh {
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(TTSH_EHIP, std::string(#expr " failed: ") + hipGetErrorString(e_));           \
+    } while (0)
+
+enum { K_RHS = 0, K_DIV = 1, K_Y = 2, K_Z = 3, K_CORR = 4, K_NKINDS = 5 };
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int div_items(int N) { return (N / kDivRows) * N; }
+
+size_t droplet_bytes(int N)
+{
+    const size_t n = (size_t)N * N * N;
+    return 20 * align256(n * sizeof(float)) + align256(n * sizeof(float2)) +
+           align256((size_t)div_items(N) * 2 * sizeof(double));
+}
+
+static int default_rhs_variant(int N) { return (N % 32 == 0) ? RHS_32x8 : RHS_16x16; }
+
+struct Slot {
+    bool used = false;
+    ttsh_params prm{};
+    int N = 0;
+    void* mem = nullptr;
+    size_t bytes = 0;
+    Drop drop{};
+    int q = 0;              // parity of the droplet's buffers: phi current = phi[q]
+    int64_t steps = 0;      // steps since set_state / admission
+    int64_t limit = -1;     // max steps, -1 = none
+    double T = 0.0;         // T accumulated in double as the reference (TTSH3D.cu:342)
+    int rhs_tz = 16, corr_tz = 16;
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int max_slots = 1;
+    size_t budget = 0, used = 0;
+    bool use_graph = false;
+    int graph_steps = 16;
+    int rhs_variant = -1;   // engine-wide K_rhs tile variant (-1: from the first droplet)
+    int cus = 256;
+
+    std::vector<Slot> slots;
+    std::map<int, float2*> tw;  // twiddle table per N
+
+    Drop* d_drops = nullptr;
+    std::vector<Drop> h_drops;
+    Work* d_work[K_NKINDS] = {};
+    int work_cap[K_NKINDS] = {};
+    int nwork[K_NKINDS] = {};
+    int* d_nwork = nullptr;
+    int maxN = 0;
+    bool tables_dirty = true;
+
+    double* d_redpart = nullptr;
+    double* d_redout = nullptr;
+    static constexpr int kRedBlocks = 256;
+
+    // graphs, one per starting parity
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gmaxN = 0;
+    int ggrid[K_NKINDS] = {};
+    int engine_parity = 0;
+
+    ~Engine() { release(); }
+
+    void drop_graphs()
+    {
+        for (int p = 0; p < 2; ++p)
+            if (gexec[p]) { (void)hipGraphExecDestroy(gexec[p]); gexec[p] = nullptr; }
+    }
+
+    void release()
+    {
+        if (st) (void)hipSetDevice(device);
+        if (st) (void)hipStreamSynchronize(st);
+        drop_graphs();
+        for (auto& s : slots)
+            if (s.mem) { (void)hipFree(s.mem); s.mem = nullptr; }
+        for (auto& kv : tw) (void)hipFree(kv.second);
+        tw.clear();
+        if (d_drops) (void)hipFree(d_drops);
+        for (int k = 0; k < K_NKINDS; ++k)
+            if (d_work[k]) { (void)hipFree(d_work[k]); d_work[k] = nullptr; }
+        if (d_nwork) (void)hipFree(d_nwork);
+        if (d_redpart) (void)hipFree(d_redpart);
+        if (d_redout) (void)hipFree(d_redout);
+        d_drops = nullptr; d_nwork = nullptr; d_redpart = nullptr; d_redout = nullptr;
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+
+    int init(int dev, int nslots, size_t bud, bool graph)
+    {
+        device = dev;
+        max_slots = nslots;
+        budget = bud;
+        use_graph = graph;
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (dev < 0 || dev >= ndev) return fail(TTSH_EHIP, "device " + std::to_string(dev) + " not present");
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        slots.resize(nslots);
+        HIPCHK(hipMalloc(&d_drops, sizeof(Drop) * nslots));
+        h_drops.resize(nslots);
+        HIPCHK(hipMalloc(&d_nwork, sizeof(int) * K_NKINDS));
+        HIPCHK(hipMalloc(&d_redpart, sizeof(double) * 3 * kRedBlocks));
+        HIPCHK(hipMalloc(&d_redout, sizeof(double) * 3));
+        return TTSH_OK;
+    }
+
+    int twiddles(int N, const float2** out)
+    {
+        auto it = tw.find(N);
+        if (it != tw.end()) { *out = it->second; return TTSH_OK; }
+        std::vector<float2> h(N);
+        const double twopi = 6.283185307179586476925286766559;
+        for (int q = 0; q < N; ++q) {
+            const double a = twopi * (double)q / (double)N;
+            h[q] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        }
+        float2* d = nullptr;
+        HIPCHK(hipMalloc(&d, sizeof(float2) * N));
+        HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+        tw[N] = d;
+        *out = d;
+        return TTSH_OK;
+    }
+
+    static void make_coef(const ttsh_params& p, Coef& co)
+    {
+        const int N = p.Nx;
+        const double h = p.h;
+        co.d1 = (float)(1.0 / (2.0 * h));
+        co.sw0 = (float)(4.0 / 6.0);
+        co.sw1 = (float)(1.0 / 6.0);
+        co.lc = (float)(1.0 / (12.0 * h * h));
+        co.lf = (float)(2.0 / (3.0 * h * h));
+        co.l0 = (float)(-14.0 / (3.0 * h * h));
+        co.Dt = (float)p.dt0;
+        co.inv_dt0 = (float)(1.0 / p.dt0);
+        co.mu = (float)p.mu;
+        co.cA = (float)p.cA;
+        co.alpha = (float)p.alpha;
+        co.beta = (float)p.beta;
+        co.eta = (float)p.eta;
+        co.La2 = (float)(p.La * p.La);
+        co.lambda0 = (float)p.lambda0;
+        co.aPhi2 = (float)(p.aPhi * 2.0);
+        co.kPhi = (float)p.kPhi;
+        co.gamma = (float)p.gamma;
+        co.A = (float)p.A;
+        const double kk = 6.283185307179586476925286766559 / (N * h);
+        co.k2s = (float)(kk * kk);
+        co.inv_n3 = (float)(1.0 / ((double)N * N * N));
+        co.anchoring = (p.A != 0.0) ? 1 : 0;
+    }
+
+    int alloc_slot(int si, const ttsh_params& p)
+    {
+        Slot& s = slots[si];
+        const int N = p.Nx;
+        const size_t bytes = droplet_bytes(N);
+        if (budget && used + bytes > budget)
+            return fail(TTSH_EBUDGET, "droplet of N=" + std::to_string(N) + " needs " + std::to_string(bytes) +
+                                          " bytes; budget has " + std::to_string(budget - used) + " left");
+        HIPCHK(hipSetDevice(device));
+        void* mem = nullptr;
+        hipError_t e = hipMalloc(&mem, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(TTSH_ENOMEM, std::string("hipMalloc of droplet state failed: ") + hipGetErrorString(e));
+        }
+        s.used = true;
+        s.prm = p;
+        s.N = N;
+        s.mem = mem;
+        s.bytes = bytes;
+        used += bytes;
+        const size_t n = (size_t)N * N * N;
+        char* b = static_cast<char*>(mem);
+        auto take = [&](size_t sz) {
+            char* r = b;
+            b += align256(sz);
+            return r;
+        };
+        Drop& D = s.drop;
+        std::memset(&D, 0, sizeof(D));
+        D.N = N;
+        D.nst = fft_plan(N, D.rad, kMaxStages);
+        const float2* t = nullptr;
+        int rc = twiddles(N, &t);
+        if (rc) return rc;
+        D.tw = t;
+        D.phi[0] = (float*)take(n * 4);
+        D.phi[1] = (float*)take(n * 4);
+        for (int a = 0; a < 3; ++a) D.P[a] = (float*)take(n * 4);
+        for (int a = 0; a < 3; ++a) D.v[a] = (float*)take(n * 4);
+        for (int q = 0; q < 2; ++q)
+            for (int a = 0; a < 3; ++a) D.vs[q][a] = (float*)take(n * 4);
+        for (int q = 0; q < 2; ++q)
+            for (int a = 0; a < 3; ++a) D.ps[q][a] = (float*)take(n * 4);
+        D.c = (float2*)take(n * 8);
+        D.npart = div_items(N);
+        D.part = (double*)take((size_t)D.npart * 16);
+        D.faithful = p.faithful_halo ? 1 : 0;
+        make_coef(p, D.co);
+        HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
+        HIPCHK(hipStreamSynchronize(st));
+        s.q = engine_parity;
+        s.steps = 0;
+        s.limit = -1;
+        s.T = p.T0;
+        s.rhs_tz = p.rhs_tz > 0 ? p.rhs_tz : 16;
+        s.corr_tz = p.corr_tz > 0 ? p.corr_tz : 16;
+        if (rhs_variant < 0) rhs_variant = p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N);
+        int tx, ty;
+        rhs_tile(rhs_variant, &tx, &ty);
+        if (N < tx + 2 || N < ty + 2) return fail(TTSH_EUNSUPPORTED, "grid smaller than the K_rhs tile apron");
+        tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    int free_slot(int si)
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipFree(s.mem));
+        used -= s.bytes;
+        s = Slot();
+        tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    // Make every droplet's buffer parity equal the engine parity by swapping
+    // its ping-pong pointers (no data moves).
+    void normalise_parity()
+    {
+        for (auto& s : slots) {
+            if (!s.used || s.q == engine_parity) continue;
+            std::swap(s.drop.phi[0], s.drop.phi[1]);
+            for (int a = 0; a < 3; ++a) {
+                std::swap(s.drop.vs[0][a], s.drop.vs[1][a]);
+                std::swap(s.drop.ps[0][a], s.drop.ps[1][a]);
+            }
+            s.q = engine_parity;
+            tables_dirty = true;
+        }
+    }
+
+    int64_t remaining(const Slot& s) const { return s.limit < 0 ? INT64_MAX : std::max<int64_t>(0, s.limit - s.steps); }
+
+    // Rebuild descriptor and work tables from the active slots.
+    int rebuild(int steps_this_call)
+    {
+        std::vector<Work> w[K_NKINDS];
+        int newmax = 0;
+        int tx, ty;
+        rhs_tile(rhs_variant < 0 ? 0 : rhs_variant, &tx, &ty);
+        for (int si = 0; si < (int)slots.size(); ++si) {
+            Slot& s = slots[si];
+            h_drops[si] = s.drop;
+            if (!s.used) continue;
+            const int64_t rem = remaining(s);
+            h_drops[si].nsteps = (int)std::min<int64_t>(rem, (int64_t)steps_this_call);
+            if (rem <= 0) continue;
+            const int N = s.N;
+            newmax = std::max(newmax, N);
+            for (int zb = 0; zb < N; zb += s.rhs_tz)
+                for (int by = 0; by < (N + ty - 1) / ty; ++by)
+                    for (int bx = 0; bx < (N + tx - 1) / tx; ++bx)
+                        w[K_RHS].push_back(Work{si, bx, by, zb | (std::min(zb + s.rhs_tz, N) << 16)});
+            for (int z = 0; z < N; ++z)
+                for (int yb = 0; yb < N / kDivRows; ++yb) w[K_DIV].push_back(Work{si, yb, z, 0});
+            for (int z = 0; z < N; ++z)
+                for (int xb = 0; xb < N / kColTile; ++xb) w[K_Y].push_back(Work{si, xb, z, 0});
+            for (int y = 0; y < N; ++y)
+                for (int xb = 0; xb < N / kColTile; ++xb) w[K_Z].push_back(Work{si, xb, y, 0});
+            for (int zb = 0; zb < N; zb += s.corr_tz)
+                for (int yb = 0; yb < N / kCorrRows; ++yb)
+                    w[K_CORR].push_back(Work{si, yb, zb | (std::min(zb + s.corr_tz, N) << 16), 0});
+        }
+        bool realloc = false;
+        for (int k = 0; k < K_NKINDS; ++k) {
+            const int cnt = (int)w[k].size();
+            if (cnt > work_cap[k]) {
+                HIPCHK(hipStreamSynchronize(st));
+                if (d_work[k]) HIPCHK(hipFree(d_work[k]));
+                const int cap = std::max(cnt, 1024);
+                HIPCHK(hipMalloc(&d_work[k], sizeof(Work) * cap));
+                work_cap[k] = cap;
+                realloc = true;
+            }
+            if (cnt) HIPCHK(hipMemcpyAsync(d_work[k], w[k].data(), sizeof(Work) * cnt, hipMemcpyHostToDevice, st));
+            nwork[k] = cnt;
+        }
+        HIPCHK(hipMemcpyAsync(d_nwork, nwork, sizeof(int) * K_NKINDS, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_drops, h_drops.data(), sizeof(Drop) * slots.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // host tables must outlive the copies
+        if (realloc || newmax > gmaxN) drop_graphs();
+        maxN = newmax;
+        tables_dirty = false;
+        return TTSH_OK;
+    }
+
+    // One step's six launches.  grid[k] <= 0 means "one work-group per item".
+    int launch_step(int parity, int kstep, const int* grid)
+    {
+        auto g = [&](int k) { return grid ? grid[k] : nwork[k]; };
+        const int lds_n = grid ? gmaxN : maxN;
+        HIPCHK(launch_rhs(rhs_variant, d_drops, d_work[K_RHS], d_nwork + K_RHS, parity, kstep, g(K_RHS), st));
+        HIPCHK(launch_divx(d_drops, d_work[K_DIV], d_nwork + K_DIV, parity, kstep, lds_n, g(K_DIV), st));
+        HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 0, kstep, lds_n, g(K_Y), st));
+        HIPCHK(launch_fftz(d_drops, d_work[K_Z], d_nwork + K_Z, kstep, lds_n, g(K_Z), st));
+        HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 1, kstep, lds_n, g(K_Y), st));
+        HIPCHK(launch_corrx(d_drops, d_work[K_CORR], d_nwork + K_CORR, parity, kstep, lds_n, g(K_CORR), st));
+        return TTSH_OK;
+    }
+
+    int capture(int parity)
+    {
+        // fixed grids: enough work-groups to fill the device, never more than
+        // the tables can hold
+        gmaxN = kMaxN;
+        for (int k = 0; k < K_NKINDS; ++k) ggrid[k] = std::max(1, std::min(work_cap[k], cus * 8));
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = TTSH_OK;
+        for (int k = 0; k < graph_steps && rc == TTSH_OK; ++k) rc = launch_step(parity ^ (k & 1), k, ggrid);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(TTSH_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&gexec[parity], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(TTSH_EHIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+        return TTSH_OK;
+    }
+
+    int any_active() const
+    {
+        int a = 0;
+        for (auto& s : slots)
+            if (s.used && remaining(s) > 0) ++a;
+        return a;
+    }
+
+    void advance(int64_t k)
+    {
+        for (auto& s : slots) {
+            if (!s.used) continue;
+            const int64_t d = std::min<int64_t>(remaining(s), k);
+            s.steps += d;
+            for (int64_t i = 0; i < d; ++i) s.T = s.T + s.prm.dt0;  // T = T + Dt (TTSH3D.cu:342)
+            if (d & 1) s.q ^= 1;
+        }
+        if (k & 1) engine_parity ^= 1;
+    }
+
+    int step(int64_t n, bool graphs)
+    {
+        HIPCHK(hipSetDevice(device));
+        while (n > 0) {
+            if (!any_active()) return TTSH_OK;
+            normalise_parity();
+            const bool g = graphs && use_graph && n >= graph_steps;
+            const int64_t chunk = g ? graph_steps : std::min<int64_t>(n, 1 << 20);
+            // per-droplet step counts of this chunk live in the descriptors
+            int rc = rebuild((int)chunk);
+            if (rc) return rc;
+            if (g) {
+                if (!gexec[engine_parity]) {
+                    rc = capture(engine_parity);
+                    if (rc) return rc;
+                }
+                HIPCHK(hipGraphLaunch(gexec[engine_parity], st));
+            } else {
+                for (int64_t k = 0; k < chunk; ++k) {
+                    rc = launch_step(engine_parity ^ (int)(k & 1), (int)k, nullptr);
+                    if (rc) return rc;
+                }
+            }
+            advance(chunk);
+            n -= chunk;
+        }
+        return TTSH_OK;
+    }
+
+    int upload_state(int si, int dtype, const void* phi, const void* P, const void* v, const void* vsp,
+                     const void* psp)
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        if (dtype != TTSH_F32 && dtype != TTSH_F64) return fail(TTSH_EINVAL, "dtype must be TTSH_F32 or TTSH_F64");
+        if (!phi || !P || !v) return fail(TTSH_EINVAL, "phi, P and v are required");
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamSynchronize(st));
+        normalise_parity();
+        const size_t n = (size_t)s.N * s.N * s.N;
+        std::vector<float> tmp;
+        auto put = [&](float* dst, const void* src, size_t off) -> int {
+            const void* p = nullptr;
+            if (dtype == TTSH_F32) {
+                p = static_cast<const float*>(src) + off;
+            } else {
+                tmp.resize(n);
+                const double* d = static_cast<const double*>(src) + off;
+                for (size_t i = 0; i < n; ++i) tmp[i] = (float)d[i];
+                p = tmp.data();
+            }
+            HIPCHK(hipMemcpy(dst, p, n * sizeof(float), hipMemcpyHostToDevice));
+            return TTSH_OK;
+        };
+        const int q = s.q;
+        int rc = put(s.drop.phi[q], phi, 0);
+        for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.P[a], P, a * n);
+        for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.v[a], v, a * n);
+        // the previous step's predictors sit in the other parity
+        for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.vs[q ^ 1][a], vsp ? vsp : v, a * n);
+        for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.ps[q ^ 1][a], psp ? psp : P, a * n);
+        if (rc) return rc;
+        s.steps = 0;
+        s.T = s.prm.T0;
+        tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    int download(int si, int dtype, void* phi, void* P, void* v)
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        if (dtype != TTSH_F32 && dtype != TTSH_F64) return fail(TTSH_EINVAL, "dtype must be TTSH_F32 or TTSH_F64");
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamSynchronize(st));
+        const size_t n = (size_t)s.N * s.N * s.N;
+        std::vector<float> tmp(n);
+        auto get = [&](const float* src, void* dst, size_t off) -> int {
+            if (!dst) return TTSH_OK;
+            if (dtype == TTSH_F32) {
+                HIPCHK(hipMemcpy(static_cast<float*>(dst) + off, src, n * sizeof(float), hipMemcpyDeviceToHost));
+            } else {
+                HIPCHK(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+                double* d = static_cast<double*>(dst) + off;
+                for (size_t i = 0; i < n; ++i) d[i] = tmp[i];
+            }
+            return TTSH_OK;
+        };
+        int rc = get(s.drop.phi[s.q], phi, 0);
+        for (int a = 0; a < 3 && !rc; ++a) rc = get(s.drop.P[a], P, a * n);
+        for (int a = 0; a < 3 && !rc; ++a) rc = get(s.drop.v[a], v, a * n);
+        return rc;
+    }
+
+    int reductions(int si, double out[3])
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        HIPCHK(hipSetDevice(device));
+        Drop d = s.drop;
+        // the reduce kernel reads phi[parity]; point it at the current buffer
+        d.phi[0] = s.drop.phi[s.q];
+        HIPCHK(hipMemcpyAsync(d_drops + si, &d, sizeof(Drop), hipMemcpyHostToDevice, st));
+        HIPCHK(launch_reduce(d_drops + si, 1, 0, d_redpart, kRedBlocks, d_redout, st));
+        double raw[3];
+        HIPCHK(hipMemcpyAsync(raw, d_redout, sizeof(raw), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        tables_dirty = true;  // descriptor slot was overwritten
+        const double h = s.prm.h, h3 = h * h * h;
+        const double n = (double)s.N * s.N * s.N;
+        out[0] = raw[0] * h3;
+        out[1] = raw[1] / n;
+        out[2] = 0.5 * raw[2] * h3;
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(out[k])) return fail(TTSH_ENONFINITE, "non-finite reduction: the droplet diverged");
+        return TTSH_OK;
+    }
+};
+
+}  // namespace ttsh
+
+using namespace ttsh;
+
+struct ttsh_sim {
+    Engine eng;
+};
+
+struct ttsh_batch {
+    Engine eng;
+};
+
+extern "C" {
+
+size_t ttsh_droplet_bytes(int N) { return droplet_bytes(N); }
+
+int ttsh_create(const ttsh_params* p, int device, ttsh_sim** out)
+{
+    if (!out) return fail(TTSH_EINVAL, "null output handle");
+    *out = nullptr;
+    int rc = ttsh_validate_params(p);
+    if (rc) return rc;
+    ttsh_sim* s = new ttsh_sim();
+    rc = s->eng.init(device, 1, 0, false);
+    if (!rc) rc = s->eng.alloc_slot(0, *p);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return TTSH_OK;
+}
+
+int ttsh_destroy(ttsh_sim* s)
+{
+    delete s;
+    return TTSH_OK;
+}
+
+int ttsh_set_state(ttsh_sim* s, int dtype, const void* phi, const void* P, const void* v, const void* vstar_prev,
+                   const void* pstar_prev)
+{
+    if (!s) return fail(TTSH_EINVAL, "null handle");
+    return s->eng.upload_state(0, dtype, phi, P, v, vstar_prev, pstar_prev);
+}
+
+int ttsh_get_state(ttsh_sim* s, int dtype, void* phi, void* P, void* v)
+{
+    if (!s) return fail(TTSH_EINVAL, "null handle");
+    return s->eng.download(0, dtype, phi, P, v);
+}
+
+int ttsh_step(ttsh_sim* s, int64_t n)
+{
+    if (!s) return fail(TTSH_EINVAL, "null handle");
+    if (n < 0) return fail(TTSH_EINVAL, "negative step count");
+    return s->eng.step(n, false);
+}
+
+int ttsh_synchronize(ttsh_sim* s)
+{
+    if (!s) return fail(TTSH_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(s->eng.device));
+    HIPCHK(hipStreamSynchronize(s->eng.st));
+    return TTSH_OK;
+}
+
+int ttsh_reductions(ttsh_sim* s, double out[3])
+{
+    if (!s || !out) return fail(TTSH_EINVAL, "null argument");
+    return s->eng.reductions(0, out);
+}
+
+int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T)
+{
+    if (!s) return fail(TTSH_EINVAL, "null handle");
+    if (steps) *steps = s->eng.slots[0].steps;
+    if (T) *T = s->eng.slots[0].T;
+    return TTSH_OK;
+}
+
+int ttsh_get_aux(ttsh_sim* s, int which, float* out)
+{
+    if (!s || !out) return fail(TTSH_EINVAL, "null argument");
+    Engine& e = s->eng;
+    Slot& sl = e.slots[0];
+    HIPCHK(hipSetDevice(e.device));
+    HIPCHK(hipStreamSynchronize(e.st));
+    const size_t n = (size_t)sl.N * sl.N * sl.N;
+    const int last = sl.q ^ 1;  // parity of the last step's predictor
+    if (which == 0 || which == 1) {
+        for (int a = 0; a < 3; ++a)
+            HIPCHK(hipMemcpy(out + a * n, which == 0 ? sl.drop.vs[last][a] : sl.drop.ps[last][a], n * sizeof(float),
+                             hipMemcpyDeviceToHost));
+        return TTSH_OK;
+    }
+    if (which == 2) {
+        HIPCHK(hipMemcpy(out, sl.drop.c, n * sizeof(float2), hipMemcpyDeviceToHost));
+        return TTSH_OK;
+    }
+    return fail(TTSH_EINVAL, "which must be 0, 1 or 2");
+}
+
+int ttsh_write_conf(ttsh_sim* s, const char* path)
+{
+    if (!s || !path) return fail(TTSH_EINVAL, "null argument");
+    const int N = s->eng.slots[0].N;
+    const size_t n = (size_t)N * N * N;
+    std::vector<float> phi(n), P(3 * n), v(3 * n);
+    int rc = s->eng.download(0, TTSH_F32, phi.data(), P.data(), v.data());
+    if (rc) return rc;
+    return write_conf_host(path, N, s->eng.slots[0].prm.ExpoData, phi.data(), P.data(), v.data());
+}
+
+int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done)
+{
+    if (!s || !run_dir) return fail(TTSH_EINVAL, "null argument");
+    Slot& sl = s->eng.slots[0];
+    const ttsh_params& p = sl.prm;
+    const std::string dir(run_dir);
+    int rc;
+    if (p.T0 == 0) {
+        rc = ttsh_write_conf(s, (dir + "/conf_0.dat").c_str());
+        if (rc) return rc;
+    }
+    // evolve() (TTSH3D.cu:311-331): step, T += Dt, export on a dte crossing,
+    // stop once T > Ts.  The export schedule is replayed on the host in the
+    // same double arithmetic, so steps between exports run as one batch.
+    double T = sl.T;
+    const double Dt = p.dt0;
+    int64_t total = 0, pending = 0;
+    for (;;) {
+        T = T + Dt;
+        ++pending;
+        const bool expo = std::floor(T / p.dte) > std::floor((T - Dt) / p.dte);
+        const bool stop = T > p.Ts;
+        if (expo || stop) {
+            rc = ttsh_step(s, pending);
+            if (rc) return rc;
+            total += pending;
+            pending = 0;
+            if (expo) {
+                const int te = (int)std::floor(T / p.dte);
+                rc = ttsh_write_conf(s, (dir + "/conf_" + std::to_string(te) + ".dat").c_str());
+                if (rc) return rc;
+            }
+            if (stop) break;
+        }
+    }
+    if (steps_done) *steps_done = total;
+    return ttsh_synchronize(s);
+}
+
+// ---- batches -----------------------------------------------------------------
+
+int ttsh_batch_create(int device, int max_slots, size_t hbm_budget, int use_graph, ttsh_batch** out)
+{
+    if (!out) return fail(TTSH_EINVAL, "null output handle");
+    *out = nullptr;
+    if (max_slots < 1 || max_slots > 4096) return fail(TTSH_EINVAL, "max_slots must be in [1, 4096]");
+    ttsh_batch* b = new ttsh_batch();
+    int rc = b->eng.init(device, max_slots, hbm_budget, use_graph != 0);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return TTSH_OK;
+}
+
+int ttsh_batch_destroy(ttsh_batch* b)
+{
+    delete b;
+    return TTSH_OK;
+}
+
+int ttsh_batch_admit(ttsh_batch* b, const ttsh_params* p, int dtype, const void* phi, const void* P, const void* v,
+                     const void* vstar_prev, const void* pstar_prev, int* slot)
+{
+    if (!b || !slot) return fail(TTSH_EINVAL, "null argument");
+    int rc = ttsh_validate_params(p);
+    if (rc) return rc;
+    Engine& e = b->eng;
+    int si = -1;
+    for (int i = 0; i < (int)e.slots.size(); ++i)
+        if (!e.slots[i].used) { si = i; break; }
+    if (si < 0) return fail(TTSH_EFULL, "no free batch slot");
+    rc = e.alloc_slot(si, *p);
+    if (rc) {
+        if (e.slots[si].used) (void)e.free_slot(si);
+        return rc;
+    }
+    rc = e.upload_state(si, dtype, phi, P, v, vstar_prev, pstar_prev);
+    if (rc) {
+        (void)e.free_slot(si);
+        return rc;
+    }
+    *slot = si;
+    return TTSH_OK;
+}
+
+static int check_slot(ttsh_batch* b, int slot)
+{
+    if (!b) return fail(TTSH_EINVAL, "null handle");
+    if (slot < 0 || slot >= (int)b->eng.slots.size() || !b->eng.slots[slot].used)
+        return fail(TTSH_EINVAL, "invalid slot " + std::to_string(slot));
+    return TTSH_OK;
+}
+
+int ttsh_batch_step(ttsh_batch* b, int64_t n)
+{
+    if (!b) return fail(TTSH_EINVAL, "null handle");
+    if (n < 0) return fail(TTSH_EINVAL, "negative step count");
+    return b->eng.step(n, true);
+}
+
+int ttsh_batch_set_limit(ttsh_batch* b, int slot, int64_t max_steps)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    b->eng.slots[slot].limit = max_steps < 0 ? -1 : max_steps;
+    b->eng.tables_dirty = true;
+    return TTSH_OK;
+}
+
+int ttsh_batch_steps_done(ttsh_batch* b, int slot, int64_t* steps)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (steps) *steps = b->eng.slots[slot].steps;
+    return TTSH_OK;
+}
+
+int ttsh_batch_get_state(ttsh_batch* b, int slot, int dtype, void* phi, void* P, void* v)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    return b->eng.download(slot, dtype, phi, P, v);
+}
+
+int ttsh_batch_reductions(ttsh_batch* b, int slot, double out[3])
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!out) return fail(TTSH_EINVAL, "null output");
+    return b->eng.reductions(slot, out);
+}
+
+int ttsh_batch_retire(ttsh_batch* b, int slot)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    return b->eng.free_slot(slot);
+}
+
+int ttsh_batch_usage(ttsh_batch* b, size_t* bytes_used, int* active)
+{
+    if (!b) return fail(TTSH_EINVAL, "null handle");
+    if (bytes_used) *bytes_used = b->eng.used;
+    if (active) {
+        int a = 0;
+        for (auto& s : b->eng.slots)
+            if (s.used) ++a;
+        *active = a;
+    }
+    return TTSH_OK;
+}
+
+int ttsh_batch_synchronize(ttsh_batch* b)
+{
+    if (!b) return fail(TTSH_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->eng.device));
+    HIPCHK(hipStreamSynchronize(b->eng.st));
+    return TTSH_OK;
+}
+
+}  // extern "C"

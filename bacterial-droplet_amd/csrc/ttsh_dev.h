@@ -1,0 +1,80 @@
+// ttsh_dev.h -- device-side data layout shared by the TTSH3D step kernels.
+//
+// One droplet lives in HBM as unpadded FP32 structure-of-arrays, x fastest and
+// z slowest: idx = (z*N + y)*N + x, the interior ordering of the reference's
+// padded fields (TTSH3D.cu:477 without the Nb halo).  Periodic boundaries are
+// index wraps inside the tile loaders; there are no halo arrays and no halo
+// kernels (the reference's BounPeriU, TTSH3D.cu:1422-1607, has no counterpart).
+//
+// Buffers per droplet (N^3 each unless stated):
+//   phi[2]           phase field, ping-pong (K_rhs reads one, writes the other)
+//   P[3], v[3]       polarization and flow (updated in place by K_corr)
+//   vs[2][3],ps[2][3] predictors v*, P*, ping-pong by step parity; the other
+//                    parity holds the previous step's v*, P*, which the
+//                    divergence reads across the periodic boundary exactly as
+//                    the reference's stale halo does (TTSH3D.cu:366-371,405-407,665-667)
+//   c                complex FP32 N^3: (div v* + i div P*)/dt0, transformed in place
+//
+// All kernels take a device table of droplet descriptors and a work table of
+// (droplet, tile...) entries, so one launch serves one droplet or a batch of
+// droplets of different sizes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttsh {
+
+constexpr int kMaxStages = 8;
+constexpr int kMaxN = 256;
+
+// FP32 coefficients of one droplet, computed in FP64 on the host.
+struct Coef {
+    float d1;         // 1/(2h): first-derivative prefactor (TTSH3D.cu:3287)
+    float sw0, sw1;   // transverse smoothing S=[sw1,sw0,sw1] = [1,4,1]/6
+    float lc, lf, l0; // Laplacian: 8 corners 1/(12h^2), 6 faces 2/(3h^2), centre -14/(3h^2)
+    float Dt;         // dt0 (Dt never changes in the reference, TTSH3D.cu:759)
+    float inv_dt0;    // 1/dt0, the getStream type-0 divisor (TTSH3D.cu:665)
+    float mu, cA, alpha, beta, eta, La2, lambda0, aPhi2, kPhi, gamma, A;
+    float k2s;        // (2 pi/(N h))^2: k2 = k2s*(mx^2+my^2+mz^2) (TTSH3D.cu:1094-1114)
+    float inv_n3;     // 1/N^3, the type-2 divisor (TTSH3D.cu:687)
+    int   anchoring;  // A != 0: evaluate the anchoring term
+};
+
+struct Drop {
+    int N;
+    int nst;                 // FFT stages
+    int rad[kMaxStages];     // radices (4, 2 or 3), product N
+    const float2* tw;        // exp(-2 pi i q/N), q in [0,N)
+    float* phi[2];
+    float* P[3];
+    float* v[3];
+    float* vs[2][3];
+    float* ps[2][3];
+    float2* c;
+    double* part;            // FP64 partial sums of c (one per divergence work item)
+    int npart;
+    int faithful;            // 1: stale-halo divergence (the reference's semantics)
+    int nsteps;              // steps this droplet takes in the current launch sequence
+    Coef co;
+};
+
+// Work item: droplet index and up to three tile coordinates.  Kernels skip an
+// item whose droplet has nsteps <= kstep (the step index within a launch
+// sequence), so droplets with different step budgets share one sequence.
+struct Work {
+    int d, a, b, c;
+};
+
+__device__ __forceinline__ int wrapi(int i, int N)
+{
+    i += (i < 0) ? N : 0;
+    i -= (i >= N) ? N : 0;
+    return i;
+}
+
+__device__ __forceinline__ size_t gidx(int N, int x, int y, int z)
+{
+    return ((size_t)z * (size_t)N + (size_t)y) * (size_t)N + (size_t)x;
+}
+
+}  // namespace ttsh

@@ -1,0 +1,322 @@
+// ttsh_host.cpp -- host-only parts of libttsh3d: parameters, input.dat I/O,
+// the conf_<m>.dat snapshot writer and the seeded initial conditions.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ttsh3d.h"
+#include "ttsh_internal.h"
+
+namespace ttsh {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+// Factor N for the device FFT: radix 4 first, then 2, then 3.  Returns the
+// number of stages, or 0 when N has another prime factor.
+int fft_plan(int N, int* rad, int maxst)
+{
+    int m = N, n = 0;
+    while (m % 4 == 0 && n < maxst) { rad[n++] = 4; m /= 4; }
+    while (m % 2 == 0 && n < maxst) { rad[n++] = 2; m /= 2; }
+    while (m % 3 == 0 && n < maxst) { rad[n++] = 3; m /= 3; }
+    return (m == 1) ? n : 0;
+}
+
+bool supported_N(int N, std::string* why)
+{
+    int rad[16];
+    if (N < 32 || N > 256 || N % 16 != 0) {
+        if (why) *why = "N must be a multiple of 16 in [32, 256], got " + std::to_string(N);
+        return false;
+    }
+    const int nst = fft_plan(N, rad, 16);
+    if (nst == 0 || nst > 8) {
+        if (why) *why = "N must have no prime factor other than 2 and 3, got " + std::to_string(N);
+        return false;
+    }
+    int rmin = 4;
+    for (int i = 0; i < nst; ++i) rmin = rad[i] < rmin ? rad[i] : rmin;
+    // butterflies per stage of the 16-line Y/Z passes must fit 4 per thread of 256
+    if (16 * (N / rmin) > 4 * 256) {
+        if (why) *why = "N = " + std::to_string(N) + " exceeds the FFT work-group capacity";
+        return false;
+    }
+    return true;
+}
+
+// SplitMix64 (Steele, Lea & Flood, OOPSLA 2014): a counter-based stream.
+static inline uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Uniform in (0,1) for (seed, cell, stream).
+static inline double unif(uint64_t seed, uint64_t cell, uint64_t k)
+{
+    const uint64_t u = splitmix64(splitmix64(seed) ^ (cell * 64ull + k));
+    return ((double)(u >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Standard normal by Box-Muller from streams (k, k+1).
+static inline double normal(uint64_t seed, uint64_t cell, uint64_t k)
+{
+    const double u1 = unif(seed, cell, k), u2 = unif(seed, cell, k + 1);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586476925286766559 * u2);
+}
+
+}  // namespace ttsh
+
+using namespace ttsh;
+
+extern "C" {
+
+const char* ttsh_last_error(void) { return g_last_error.c_str(); }
+
+const char* ttsh_version(void) { return "ttsh3d-mi355x 0.1"; }
+
+int ttsh_params_default(ttsh_params* p)
+{
+    if (!p) return fail(TTSH_EINVAL, "null params");
+    std::memset(p, 0, sizeof(*p));
+    // run_cuda.pl:7-44
+    p->TimeScheme = 1;
+    p->RKStage = 16;
+    p->ExplImpi = 1;
+    p->AdapTime = 0;
+    p->ExpoData = 1;
+    p->InitCond = 1;
+    p->isSimplified = 1;
+    p->checkUnStable = 0;
+    p->Nx = p->Ny = p->Nz = 64;
+    p->Nb = 3;
+    p->h = 1.0;
+    p->dt0 = 0.002;
+    p->T0 = 0.0;
+    p->Ts = 1000.0;
+    p->dte = 10.0;
+    p->dtVarC = 0.01;
+    p->dtMax = 0.000005;
+    p->mu = 1.0;
+    p->cA = -1.0;
+    p->alpha = -0.02;
+    p->beta = 4.366;
+    p->eta = -0.187;
+    p->La = 1.0;
+    p->kappa = -0.364;
+    p->lambda0 = 9.0;
+    p->aPhi = 1.0;
+    p->kPhi = 1.0;
+    p->gamma = 1.0;
+    p->A = 0.0;
+    p->init0 = 0.0001;
+    p->ReadSuccess = -8848;
+    p->faithful_halo = 1;
+    p->rhs_variant = -1;
+    p->rhs_tz = 0;
+    p->corr_tz = 0;
+    return TTSH_OK;
+}
+
+int ttsh_parse_input_dat(const char* path, ttsh_params* p)
+{
+    if (!path || !p) return fail(TTSH_EINVAL, "null argument");
+    std::ifstream in(path);
+    if (!in.is_open()) return fail(TTSH_EIO, std::string("cannot open ") + path);
+    ttsh_params q = *p;
+    q.ReadSuccess = 0;
+    // Read order of GetInput, TTSH3D.cu:702-739.
+    in >> q.TimeScheme >> q.RKStage >> q.ExplImpi >> q.AdapTime >> q.ExpoData >> q.InitCond >> q.isSimplified >>
+        q.checkUnStable >> q.Nx >> q.Ny >> q.Nz >> q.Nb >> q.h >> q.dt0 >> q.T0 >> q.Ts >> q.dte >> q.dtVarC >>
+        q.dtMax >> q.mu >> q.cA >> q.alpha >> q.beta >> q.eta >> q.La >> q.kappa >> q.lambda0 >> q.aPhi >> q.kPhi >>
+        q.gamma >> q.A >> q.init0;
+    if (!in) return fail(TTSH_EIO, std::string("input file is short or malformed: ") + path);
+    in >> q.ReadSuccess;
+    if (!in || q.ReadSuccess != -8848) {
+        *p = q;
+        return fail(TTSH_ESENTINEL, "Error while reading the input file!");
+    }
+    *p = q;
+    return TTSH_OK;
+}
+
+int ttsh_write_input_dat(const char* path, const ttsh_params* p)
+{
+    if (!path || !p) return fail(TTSH_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(TTSH_EIO, std::string("cannot write ") + path);
+    // Perl interpolates numbers with %.15g; write order run_cuda.pl:54-86.
+    std::fprintf(f, "%d\n%d\n%d\n%d\n%d\n%d\n%d\n%d\n", p->TimeScheme, p->RKStage, p->ExplImpi, p->AdapTime,
+                 p->ExpoData, p->InitCond, p->isSimplified, p->checkUnStable);
+    std::fprintf(f, "%d\n%d\n%d\n%d\n", p->Nx, p->Ny, p->Nz, p->Nb);
+    const double d[] = {p->h,  p->dt0,   p->T0,    p->Ts,    p->dte,   p->dtVarC, p->dtMax, p->mu,
+                        p->cA, p->alpha, p->beta,  p->eta,   p->La,    p->kappa,  p->lambda0, p->aPhi,
+                        p->kPhi, p->gamma, p->A, p->init0};
+    for (double v : d) std::fprintf(f, "%.15g\n", v);
+    std::fprintf(f, "%d\n", p->ReadSuccess);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? TTSH_OK : fail(TTSH_EIO, std::string("write failed: ") + path);
+}
+
+int ttsh_validate_params(const ttsh_params* p)
+{
+    if (!p) return fail(TTSH_EINVAL, "null params");
+    if (p->ReadSuccess != -8848) return fail(TTSH_ESENTINEL, "ReadSuccess sentinel is not -8848");
+    if (p->TimeScheme != 1)
+        return fail(TTSH_EUNSUPPORTED, "only TimeScheme 1 (explicit Euler) evolves the fields (TTSH3D.cu:338-341)");
+    if (p->Nx != p->Ny || p->Nx != p->Nz)
+        return fail(TTSH_EUNSUPPORTED, "grid must be cubic (Nx == Ny == Nz)");
+    std::string why;
+    if (!supported_N(p->Nx, &why)) return fail(TTSH_EUNSUPPORTED, why);
+    if (p->Nb < 1) return fail(TTSH_EINVAL, "Nb must be >= 1");
+    if (!(p->h > 0.0) || !(p->dt0 > 0.0)) return fail(TTSH_EINVAL, "h and dt0 must be positive");
+    if (!(p->dte > 0.0)) return fail(TTSH_EINVAL, "dte must be positive");
+    if (p->rhs_variant < -1 || p->rhs_variant >= 4) return fail(TTSH_EINVAL, "rhs_variant out of range");
+    if (p->rhs_tz < 0 || p->corr_tz < 0) return fail(TTSH_EINVAL, "tile depths must be >= 0");
+    return TTSH_OK;
+}
+
+int ttsh_init_reference(const ttsh_params* p, uint64_t seed, double* phi, double* P, double* v, double* vstar,
+                        double* pstar)
+{
+    if (!p || !phi || !P || !v) return fail(TTSH_EINVAL, "null argument");
+    if (p->Nx != p->Ny || p->Nx != p->Nz) return fail(TTSH_EUNSUPPORTED, "grid must be cubic");
+    const int N = p->Nx;
+    const size_t n = (size_t)N * N * N;
+    const double a = p->init0;
+    // InitConf, InitCond == 1 (TTSH3D.cu:798-825): i is x, j is y, k is z.
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                const size_t idx = ((size_t)k * N + j) * N + i;
+                const uint64_t c = idx;
+                v[idx] = 0 + a * (unif(seed, c, 0) - 0.5);
+                v[n + idx] = 0 + a * (unif(seed, c, 1) - 0.5);
+                v[2 * n + idx] = 0 + a * (unif(seed, c, 2) - 0.5);
+                if (vstar) {
+                    vstar[idx] = 0 + a * (unif(seed, c, 3) - 0.5);
+                    vstar[n + idx] = 0 + a * (unif(seed, c, 4) - 0.5);
+                    vstar[2 * n + idx] = 0 + a * (unif(seed, c, 5) - 0.5);
+                }
+                P[idx] = 0 + a * (unif(seed, c, 6) - 0.5);
+                P[n + idx] = 0 + a * (unif(seed, c, 7) - 0.5);
+                P[2 * n + idx] = 0 + a * (unif(seed, c, 8) - 0.5);
+                if (pstar) {
+                    pstar[idx] = 0 + a * (unif(seed, c, 9) - 0.5);
+                    pstar[n + idx] = 0 + a * (unif(seed, c, 10) - 0.5);
+                    pstar[2 * n + idx] = 0 + a * (unif(seed, c, 11) - 0.5);
+                }
+                const double dx = (i - p->Nx / 2.);
+                const double dy = (j - p->Ny / 2.);
+                const double dz = (k - p->Nz / 2.);
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                phi[idx] = (1 - std::tanh((r - p->Nx / 4.) * 2)) / 2;
+            }
+    return TTSH_OK;
+}
+
+int ttsh_init_droplet(int N, double R, double xi, int pmode, uint64_t seed, double* phi, double* P, double* v)
+{
+    if (!phi || !P || !v) return fail(TTSH_EINVAL, "null argument");
+    if (N <= 0 || !(xi > 0.0) || !(R > 0.0)) return fail(TTSH_EINVAL, "need N > 0, R > 0, xi > 0");
+    if (pmode != 0 && pmode != 1) return fail(TTSH_EINVAL, "pmode must be 0 (random unit) or 1 (aster)");
+    const size_t n = (size_t)N * N * N;
+    const double c0 = N / 2.0;
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                const size_t idx = ((size_t)z * N + y) * N + x;
+                const double dx = x - c0, dy = y - c0, dz = z - c0;
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                const double ph = 0.5 * (1.0 - std::tanh((r - R) / xi));
+                phi[idx] = ph;
+                v[idx] = v[n + idx] = v[2 * n + idx] = 0.0;
+                double px = 0, py = 0, pz = 0;
+                if (ph > 0.5) {
+                    const double g0 = normal(seed, idx, 0), g1 = normal(seed, idx, 2), g2 = normal(seed, idx, 4);
+                    if (pmode == 0) {
+                        px = g0; py = g1; pz = g2;
+                    } else {
+                        const double inv = r > 0 ? 1.0 / r : 0.0;
+                        px = dx * inv + 0.1 * g0;
+                        py = dy * inv + 0.1 * g1;
+                        pz = dz * inv + 0.1 * g2;
+                    }
+                    const double nrm = std::sqrt(px * px + py * py + pz * pz);
+                    if (nrm > 0) { px /= nrm; py /= nrm; pz /= nrm; }
+                }
+                P[idx] = px;
+                P[n + idx] = py;
+                P[2 * n + idx] = pz;
+            }
+    return TTSH_OK;
+}
+
+}  // extern "C"
+
+namespace ttsh {
+
+// Format a double as std::fixed << setprecision(15) does (printf "%.15f").
+static inline int fmt15(char* out, double v) { return std::snprintf(out, 64, "%.15f", v); }
+
+int write_conf_host(const char* path, int N, int expo_data, const float* phi, const float* P, const float* v)
+{
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(TTSH_EIO, std::string("cannot write ") + path);
+    const size_t n = (size_t)N * N * N;
+    std::vector<char> buf;
+    buf.reserve((size_t)N * 8 * 24 + 64);
+    char tmp[64];
+    // ExpoConf loop order (TTSH3D.cu:975-981): i < Ny (y), j < Nx (x), k < Nz (z).
+    for (int i = 0; i < N; ++i) {
+        for (int j = 0; j < N; ++j) {
+            buf.clear();
+            for (int k = 0; k < N; ++k) {
+                const size_t idx = ((size_t)k * N + i) * N + j;
+                if (expo_data == 1) {
+                    const float vals[7] = {v[idx], v[n + idx], v[2 * n + idx], P[idx], P[n + idx], P[2 * n + idx],
+                                           phi[idx]};
+                    for (int q = 0; q < 7; ++q) {
+                        const int m = fmt15(tmp, (double)vals[q]);
+                        buf.insert(buf.end(), tmp, tmp + m);
+                        buf.push_back(q < 6 ? ' ' : '\n');
+                    }
+                } else if (expo_data == -1) {
+                    const float vals[2] = {v[idx], v[n + idx]};
+                    for (int q = 0; q < 2; ++q) {
+                        const int m = fmt15(tmp, (double)vals[q]);
+                        buf.insert(buf.end(), tmp, tmp + m);
+                        buf.push_back(' ');
+                    }
+                }
+            }
+            if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+                std::fclose(f);
+                return fail(TTSH_EIO, std::string("write failed: ") + path);
+            }
+        }
+    }
+    if (std::fclose(f) != 0) return fail(TTSH_EIO, std::string("close failed: ") + path);
+    return TTSH_OK;
+}
+
+}  // namespace ttsh
+
+extern "C" int ttsh_write_conf_arrays(const char* path, int N, int ExpoData, const float* phi, const float* P,
+                                      const float* v)
+{
+    if (!path || !phi || !P || !v || N <= 0) return ttsh::fail(TTSH_EINVAL, "bad argument");
+    return ttsh::write_conf_host(path, N, ExpoData, phi, P, v);
+}

@@ -1,0 +1,159 @@
+"""GPU parity: the HIP step (libttsh3d.so, through the C ABI) against the FP64
+oracle restating TTSH3D.cu's EEuler (TTSH3D.cu:347-359) on the same seeded
+inputs.
+
+Tolerances (FP32 device arithmetic against FP64):
+  - one step, each stage output (v*, P*, phi, v, P, p): relative RMS <= 2e-5;
+  - 100 and 1,000 steps: relative RMS <= 1e-4 per field (the accuracy target
+    for FP32 against the FP64 oracle).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_KEYS = ("h", "dt0", "mu", "cA", "alpha", "beta", "eta", "La", "kappa", "lambda0",
+               "aPhi", "kPhi", "gamma", "A")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_for(O, p, phi, P, v, vs=None, ps=None, faithful=True):
+    o = O.Oracle(p.Nx, faithful=faithful, **{k: getattr(p, k) for k in ORACLE_KEYS})
+    o.set_state(phi, P, v, vs, ps)
+    return o
+
+
+def droplet_case(ttsh, N, seed, pmode=0, **over):
+    p = ttsh.default_params(N=N, dt0=0.01, **over)
+    phi, P, v = ttsh.init_droplet(N, 20.0 * N / 64.0, 2.0, pmode, seed)
+    return p, phi, P, v
+
+
+def compare_state(sim, o, tol, tag):
+    phi, P, v = sim.get_state()
+    ophi, oP, ov = o.get_state()
+    errs = {"phi": rel_rms(phi, ophi), "P": rel_rms(P, oP), "v": rel_rms(v, ov)}
+    log(tag, {k: f"{e:.2e}" for k, e in errs.items()},
+        "max", f"{max_rel(phi, ophi):.2e} {max_rel(P, oP):.2e} {max_rel(v, ov):.2e}")
+    for k, e in errs.items():
+        assert e <= tol, f"{tag}: {k} relative RMS {e:.3e} > {tol}"
+    return errs
+
+
+@pytest.mark.parametrize("N", [32, 48, 64])
+def test_one_step_stages(ttsh, oracle_mod, N):
+    log(f"[one_step_stages N={N}]")
+    p, phi, P, v = droplet_case(ttsh, N, seed=1000 + N)
+    o = oracle_for(oracle_mod, p, phi, P, v)
+    o.step(1)
+    with ttsh.Simulation(p) as sim:
+        sim.set_state(phi, P, v)
+        sim.step(1)
+        vs = sim.aux("vstar")
+        ps = sim.aux("pstar")
+        pv, pp = sim.aux("p")
+        e = {"vstar": rel_rms(vs, o.aux("vstar")), "pstar": rel_rms(ps, o.aux("pstar")),
+             "p_v": rel_rms(pv - pv.mean(), o.aux("p_v") - o.aux("p_v").mean()),
+             "p_p": rel_rms(pp - pp.mean(), o.aux("p_p") - o.aux("p_p").mean())}
+        log("stages", {k: f"{x:.2e}" for k, x in e.items()})
+        for k, x in e.items():
+            assert x <= 2e-5, f"{k}: {x}"
+        compare_state(sim, o, 2e-5, f"N={N} 1 step")
+        steps, T = sim.clock()
+        assert steps == 1 and T == pytest.approx(0.01)
+
+
+def test_one_step_parameters(ttsh, oracle_mod):
+    """Non-default h, activity, elasticity, anchoring (A != 0) and aster P."""
+    log("[one_step_parameters]")
+    p, phi, P, v = droplet_case(ttsh, 48, seed=7, pmode=1, h=1.25, cA=-2.5, La=1.3, A=0.4,
+                                alpha=0.05, gamma=0.8, mu=0.7, lambda0=3.0, kPhi=1.2, aPhi=0.9)
+    v = 0.05 * np.sin(np.arange(v.size).reshape(v.shape) * 0.001)
+    o = oracle_for(oracle_mod, p, phi, P, v)
+    o.step(1)
+    with ttsh.Simulation(p) as sim:
+        sim.set_state(phi, P, v)
+        sim.step(1)
+        compare_state(sim, o, 2e-5, "params 1 step")
+
+
+def test_reference_init_stale_halo(ttsh, oracle_mod):
+    """InitCond=1 state with v*, P* of the 'previous step' different from v, P:
+    the divergence must read them across the periodic boundary (faithful)."""
+    log("[reference_init_stale_halo]")
+    p = ttsh.default_params(N=32, dt0=0.005, init0=0.05)
+    phi, P, v, vs, ps = ttsh.init_reference(p, seed=11)
+    for faithful in (True, False):
+        p.faithful_halo = 1 if faithful else 0
+        o = oracle_for(oracle_mod, p, phi, P, v, vs, ps, faithful=faithful)
+        o.step(3)
+        with ttsh.Simulation(p) as sim:
+            sim.set_state(phi, P, v, vs, ps)
+            sim.step(3)
+            compare_state(sim, o, 5e-5, f"reference init faithful={faithful}")
+
+
+@pytest.mark.parametrize("N,steps", [(32, 100), (64, 100)])
+def test_multi_step(ttsh, oracle_mod, N, steps):
+    log(f"[multi_step N={N} steps={steps}]")
+    p, phi, P, v = droplet_case(ttsh, N, seed=2000 + N)
+    o = oracle_for(oracle_mod, p, phi, P, v)
+    o.step(steps)
+    with ttsh.Simulation(p) as sim:
+        sim.set_state(phi, P, v)
+        sim.step(steps)
+        compare_state(sim, o, 1e-4, f"N={N} {steps} steps")
+        red, ok = sim.reductions()
+        ored = o.reductions()
+        log("reductions gpu", red, "oracle", ored)
+        assert ok
+        assert red[0] == pytest.approx(ored[0], rel=1e-6)
+        assert red[1] == pytest.approx(ored[1], rel=1e-4)
+        assert red[2] == pytest.approx(ored[2], rel=1e-3, abs=1e-12)
+
+
+def test_thousand_steps_32(ttsh, oracle_mod):
+    log("[thousand_steps_32]")
+    p, phi, P, v = droplet_case(ttsh, 32, seed=3032)
+    o = oracle_for(oracle_mod, p, phi, P, v)
+    o.step(1000)
+    with ttsh.Simulation(p) as sim:
+        sim.set_state(phi, P, v)
+        sim.step(1000)
+        compare_state(sim, o, 1e-4, "N=32 1000 steps")
+
+
+def test_batch_alone_vs_mix(ttsh):
+    """A droplet stepped alone (plain launches), alone in a graph batch and in a
+    mix of other sizes gives bitwise the same state."""
+    log("[batch_alone_vs_mix]")
+    pa, phia, Pa, va = droplet_case(ttsh, 48, seed=5)
+    pb, phib, Pb, vb = droplet_case(ttsh, 64, seed=6, cA=-3.0)
+    pc, phic, Pc, vc = droplet_case(ttsh, 32, seed=8, La=0.8)
+    steps = 40
+    with ttsh.Simulation(pa) as sim:
+        sim.set_state(phia, Pa, va)
+        sim.step(steps)
+        ref = sim.get_state()
+    with ttsh.Batch(max_slots=4, use_graph=True) as b:
+        sa = b.admit(pa, phia, Pa, va)
+        b.step(steps)
+        alone = b.get_state(sa)
+    with ttsh.Batch(max_slots=4, use_graph=True) as b:
+        sb = b.admit(pb, phib, Pb, vb)
+        sa = b.admit(pa, phia, Pa, va)
+        sc = b.admit(pc, phic, Pc, vc)
+        b.step(steps)
+        mix = b.get_state(sa)
+        assert b.steps_done(sa) == steps and b.steps_done(sb) == steps
+    for x, y, z in zip(ref, alone, mix):
+        assert np.array_equal(x, y)
+        assert np.array_equal(x, z)
