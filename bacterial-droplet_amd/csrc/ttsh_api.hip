@@ -49,7 +49,15 @@ size_t droplet_bytes(int N)
            align256((size_t)div_items(N) * 2 * sizeof(double));
 }
 
-static int default_rhs_variant(int N) { return (N % 32 == 0) ? RHS_32x8 : RHS_16x16; }
+static int default_rhs_variant(int N) { return (N % 32 == 0 && N >= 64) ? RHS_32x8 : RHS_16x16; }
+
+// A K_rhs tile needs N >= TX+2 and N >= TY+2; 16x16 fits every supported N.
+static bool variant_fits(int v, int N)
+{
+    int tx, ty;
+    rhs_tile(v, &tx, &ty);
+    return N >= tx + 2 && N >= ty + 2;
+}
 
 struct Slot {
     bool used = false;
@@ -253,9 +261,11 @@ struct Engine {
         s.rhs_tz = p.rhs_tz > 0 ? p.rhs_tz : 16;
         s.corr_tz = p.corr_tz > 0 ? p.corr_tz : 16;
         if (rhs_variant < 0) rhs_variant = p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N);
-        int tx, ty;
-        rhs_tile(rhs_variant, &tx, &ty);
-        if (N < tx + 2 || N < ty + 2) return fail(TTSH_EUNSUPPORTED, "grid smaller than the K_rhs tile apron");
+        if (!variant_fits(rhs_variant, N)) {
+            if (p.rhs_variant >= 0) return fail(TTSH_EUNSUPPORTED, "grid smaller than the requested K_rhs tile");
+            rhs_variant = RHS_16x16;  // engine-wide: every droplet fits it
+            drop_graphs();
+        }
         tables_dirty = true;
         return TTSH_OK;
     }
@@ -344,18 +354,60 @@ struct Engine {
         return TTSH_OK;
     }
 
-    // One step's six launches.  grid[k] <= 0 means "one work-group per item".
-    int launch_step(int parity, int kstep, const int* grid)
+    // One step's six launches.  grid == nullptr means "one work-group per
+    // item".  ev (7 events, may be null) is recorded around every launch.
+    int launch_step(int parity, int kstep, const int* grid, hipEvent_t* ev = nullptr)
     {
         auto g = [&](int k) { return grid ? grid[k] : nwork[k]; };
         const int lds_n = grid ? gmaxN : maxN;
+        auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+        HIPCHK(mark(0));
         HIPCHK(launch_rhs(rhs_variant, d_drops, d_work[K_RHS], d_nwork + K_RHS, parity, kstep, g(K_RHS), st));
+        HIPCHK(mark(1));
         HIPCHK(launch_divx(d_drops, d_work[K_DIV], d_nwork + K_DIV, parity, kstep, lds_n, g(K_DIV), st));
+        HIPCHK(mark(2));
         HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 0, kstep, lds_n, g(K_Y), st));
+        HIPCHK(mark(3));
         HIPCHK(launch_fftz(d_drops, d_work[K_Z], d_nwork + K_Z, kstep, lds_n, g(K_Z), st));
+        HIPCHK(mark(4));
         HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 1, kstep, lds_n, g(K_Y), st));
+        HIPCHK(mark(5));
         HIPCHK(launch_corrx(d_drops, d_work[K_CORR], d_nwork + K_CORR, parity, kstep, lds_n, g(K_CORR), st));
+        HIPCHK(mark(6));
         return TTSH_OK;
+    }
+
+    // n plain steps with a HIP event between consecutive kernels; ms[k] gets
+    // the summed duration of kernel k over the n steps.
+    int profile(int64_t n, double ms[6])
+    {
+        HIPCHK(hipSetDevice(device));
+        for (int k = 0; k < 6; ++k) ms[k] = 0.0;
+        constexpr int64_t kChunk = 512;  // steps recorded before the events are read
+        std::vector<hipEvent_t> ev(7 * kChunk);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        int rc = TTSH_OK;
+        while (n > 0 && rc == TTSH_OK) {
+            const int64_t m = std::min(n, kChunk);
+            normalise_parity();
+            rc = rebuild((int)m);
+            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
+                rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, nullptr, &ev[7 * s]);
+            if (rc) break;
+            hipError_t e = hipEventSynchronize(ev[7 * (m - 1) + 6]);
+            if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
+            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
+                for (int k = 0; k < 6; ++k) {
+                    float t = 0.f;
+                    e = hipEventElapsedTime(&t, ev[7 * s + k], ev[7 * s + k + 1]);
+                    if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
+                    ms[k] += t;
+                }
+            advance(m);
+            n -= m;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc;
     }
 
     int capture(int parity)
@@ -590,6 +642,13 @@ int ttsh_reductions(ttsh_sim* s, double out[3])
 {
     if (!s || !out) return fail(TTSH_EINVAL, "null argument");
     return s->eng.reductions(0, out);
+}
+
+int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[6])
+{
+    if (!s || !kernel_ms) return fail(TTSH_EINVAL, "null argument");
+    if (n < 0) return fail(TTSH_EINVAL, "negative step count");
+    return s->eng.profile(n, kernel_ms);
 }
 
 int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T)

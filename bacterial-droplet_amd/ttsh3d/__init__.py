@@ -85,6 +85,7 @@ def lib():
         "ttsh_write_conf_arrays": (C.c_int, [C.c_char_p, C.c_int, C.c_int, vp, vp, vp]),
         "ttsh_run": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
         "ttsh_get_clock": (C.c_int, [vp, C.POINTER(i64), dp]),
+        "ttsh_profile": (C.c_int, [vp, i64, dp]),
         "ttsh_get_aux": (C.c_int, [vp, C.c_int, vp]),
         "ttsh_init_reference": (C.c_int, [P, C.c_uint64, vp, vp, vp, vp, vp]),
         "ttsh_init_droplet": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, vp, vp, vp]),
@@ -246,6 +247,14 @@ class Simulation:
         if rc not in (OK, ENONFINITE):
             _check(rc)
         return np.array(out[:]), rc == OK
+
+    KERNELS = ("K_rhs", "K_divx", "K_ffty_fwd", "K_fftz", "K_ffty_inv", "K_corrx")
+
+    def profile(self, n: int):
+        """Run n steps with HIP events between kernels; returns {kernel: total ms}."""
+        out = (C.c_double * 6)()
+        _check(lib().ttsh_profile(self._h, int(n), out))
+        return dict(zip(self.KERNELS, out[:]))
 
     def clock(self):
         s = C.c_int64(); T = C.c_double()

@@ -136,6 +136,13 @@ int ttsh_write_conf_arrays(const char* path, int N, int ExpoData, const float* p
  * number of steps taken. */
 int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done);
 
+/* New (the reference only prints host CPU time, TTSH3D.cu:266,284-286): runs n
+ * steps like ttsh_step with a HIP event recorded between consecutive kernels
+ * on the handle's stream; kernel_ms[k] receives the summed milliseconds of
+ * kernel k = K_rhs, K_divx, K_ffty (forward), K_fftz, K_ffty (inverse),
+ * K_corrx.  Synchronises once per 512 steps. */
+int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[6]);
+
 /* Steps taken since the last ttsh_set_state, and the time T. */
 int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T);
 
