@@ -70,7 +70,7 @@ struct Slot {
     int64_t steps = 0;      // steps since set_state / admission
     int64_t limit = -1;     // max steps, -1 = none
     double T = 0.0;         // T accumulated in double as the reference (TTSH3D.cu:342)
-    int rhs_tz = 16, corr_tz = 16;
+    int rhs_tz = 16, corr_tz = 4;
 };
 
 struct Engine {
@@ -259,7 +259,7 @@ struct Engine {
         s.limit = -1;
         s.T = p.T0;
         s.rhs_tz = p.rhs_tz > 0 ? p.rhs_tz : 16;
-        s.corr_tz = p.corr_tz > 0 ? p.corr_tz : 16;
+        s.corr_tz = p.corr_tz > 0 ? p.corr_tz : 4;
         if (rhs_variant < 0) rhs_variant = p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N);
         if (!variant_fits(rhs_variant, N)) {
             if (p.rhs_variant >= 0) return fail(TTSH_EUNSUPPORTED, "grid smaller than the requested K_rhs tile");
