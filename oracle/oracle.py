@@ -60,6 +60,7 @@ def lib():
         L.orc_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_sizeof_params.restype = C.c_size_t
+        L.orc_projection.argtypes = [C.POINTER(OrcParams), dp, dp, dp]
         assert L.orc_sizeof_params() == C.sizeof(OrcParams)
         _lib = L
     return _lib
@@ -133,7 +134,7 @@ class Oracle:
 
     _AUX = {"vstar": (0, 3), "pstar": (1, 3), "p_v": (2, 1), "p_p": (3, 1), "dtv": (4, 3),
             "dtP": (5, 3), "dtphi": (6, 1), "mu0": (7, 1), "P2": (8, 1), "mu": (9, 3),
-            "Sigma": (10, 9), "G": (11, 9)}
+            "Sigma": (10, 9), "G": (11, 9), "sdiv_v": (12, 1), "sdiv_p": (13, 1)}
 
     def aux(self, name: str):
         which, k = self._AUX[name]
@@ -158,6 +159,17 @@ def stencil(which: str, f, h: float = 1.0):
     out = np.empty_like(f)
     lib().orc_stencil({"d1x": 0, "d1y": 1, "d1z": 2, "lap": 3}[which], N, float(h), f, out)
     return out
+
+
+def projection(Xs, **params):
+    """getP + corrector on a given predictor (3,N,N,N), clean halo: returns (p, X)."""
+    Xs = _c(Xs)
+    N = Xs.shape[1]
+    prm = make_params(N, False, **params)
+    p = np.empty(Xs.shape[1:])
+    X = np.empty_like(Xs)
+    lib().orc_projection(C.byref(prm), Xs, p, X)
+    return p, X
 
 
 def fft3d(a, inverse: bool = False):

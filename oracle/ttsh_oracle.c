@@ -64,6 +64,7 @@ typedef struct {
     double *dtvx, *dtvy, *dtvz, *dtpx, *dtpy, *dtpz, *dtphi;
     double *p_v, *p_p;
     double *fre, *fim; /* Fstream */
+    double *sdiv_v, *sdiv_p; /* getStream type-0 output (div/dt0) before the FFT */
     double *k2;
 } orc_state;
 
@@ -368,7 +369,7 @@ orc_state *orc_create(const orc_params *prm)
                       &s->DPx, &s->DPy, &s->DPz, &s->OPx, &s->OPy, &s->OPz,
                       &s->Axx, &s->Axy, &s->Axz, &s->Ayy, &s->Ayz, &s->Azz,
                       &s->dtvx, &s->dtvy, &s->dtvz, &s->dtpx, &s->dtpy, &s->dtpz, &s->dtphi,
-                      &s->p_v, &s->p_p, &s->fre, &s->fim, &s->k2};
+                      &s->p_v, &s->p_p, &s->fre, &s->fim, &s->k2, &s->sdiv_v, &s->sdiv_p};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) *all[i] = zalloc(n);
     for (int i = 0; i < 9; i++) { s->G[i] = zalloc(n); s->Sig[i] = zalloc(n); }
 
@@ -403,7 +404,7 @@ void orc_destroy(orc_state *s)
                      s->DPx, s->DPy, s->DPz, s->OPx, s->OPy, s->OPz,
                      s->Axx, s->Axy, s->Axz, s->Ayy, s->Ayz, s->Azz,
                      s->dtvx, s->dtvy, s->dtvz, s->dtpx, s->dtpy, s->dtpz, s->dtphi,
-                     s->p_v, s->p_p, s->fre, s->fim, s->k2};
+                     s->p_v, s->p_p, s->fre, s->fim, s->k2, s->sdiv_v, s->sdiv_p};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(all[i]);
     for (int i = 0; i < 9; i++) { free(s->G[i]); free(s->Sig[i]); }
     free(s);
@@ -432,7 +433,8 @@ void orc_get_state(const orc_state *s, double *phi, double *P, double *v)
 }
 
 /* which: 0 v*(3n) 1 P*(3n) 2 p_v 3 p_p 4 dtv(3n) 5 dtP(3n) 6 dtphi 7 mu0 8 P2
- *        9 mu(3n) 10 Sigma(9n, xx xy xz yx yy yz zx zy zz) 11 G(9n) */
+ *        9 mu(3n) 10 Sigma(9n, xx xy xz yx yy yz zx zy zz) 11 G(9n)
+ *        12 div(v*)/dt0 13 div(P*)/dt0 (before the FFT) */
 void orc_get_aux(const orc_state *s, int which, double *out)
 {
     const size_t n = s->n, b = n * sizeof(double);
@@ -449,6 +451,8 @@ void orc_get_aux(const orc_state *s, int which, double *out)
     case 9: memcpy(out, s->mux, b); memcpy(out + n, s->muy, b); memcpy(out + 2 * n, s->muz, b); break;
     case 10: for (int i = 0; i < 9; i++) memcpy(out + i * n, s->Sig[i], b); break;
     case 11: for (int i = 0; i < 9; i++) memcpy(out + i * n, s->G[i], b); break;
+    case 12: memcpy(out, s->sdiv_v, b); break;
+    case 13: memcpy(out, s->sdiv_p, b); break;
     default: break;
     }
 }
@@ -628,8 +632,9 @@ static void get_p(orc_state *s, const orc_fftplan *pl, int fieldType)
                 s->fim[idx] = 0;
             }
         }
-    fft3d_plan(pl, s->fre, s->fim, -1);
     const size_t n = s->n;
+    memcpy(fieldType == 0 ? s->sdiv_v : s->sdiv_p, s->fre, n * sizeof(double));
+    fft3d_plan(pl, s->fre, s->fim, -1);
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) {
         if (s->k2[i] == 0) {
@@ -697,6 +702,39 @@ void orc_step(orc_state *s, int nsteps)
         get_vnew(s, Dt);
     }
     fftplan_free(&pl);
+}
+
+/* The projection alone, clean halo: p from getP (TTSH3D.cu:635-691) applied to
+ * the given predictor Xs (3n), then the corrector X = Xs - Dt D p
+ * (TTSH3D.cu:505-511).  Writes p (n) and X (3n). */
+void orc_projection(const orc_params *prm, const double *Xs, double *p_out, double *X_out)
+{
+    orc_params q = *prm;
+    q.faithful = 0;
+    orc_state *s = orc_create(&q);
+    const size_t n = s->n, b = n * sizeof(double);
+    memcpy(s->vxs, Xs, b);
+    memcpy(s->vys, Xs + n, b);
+    memcpy(s->vzs, Xs + 2 * n, b);
+    orc_fftplan pl;
+    fftplan_init(&pl, q.N);
+    get_p(s, &pl, 0);
+    fftplan_free(&pl);
+    memcpy(p_out, s->p_v, b);
+    const int N = q.N;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < N; z++)
+        for (int y = 0; y < N; y++) {
+            size_t nb[27];
+            for (int x = 0; x < N; x++) {
+                neighbourhood(N, x, y, z, nb, NULL);
+                const size_t idx = nb[13];
+                X_out[idx] = s->vxs[idx] - q.dt0 * d1x(s->p_v, nb, q.h);
+                X_out[n + idx] = s->vys[idx] - q.dt0 * d1y(s->p_v, nb, q.h);
+                X_out[2 * n + idx] = s->vzs[idx] - q.dt0 * d1z(s->p_v, nb, q.h);
+            }
+        }
+    orc_destroy(s);
 }
 
 /* FP64 diagnostics: mass = sum(phi) h^3, mean |P|, kinetic energy = 1/2 sum |v|^2 h^3 */

@@ -1,0 +1,226 @@
+"""Host-side tests of libttsh3d's C ABI (no GPU calls).
+
+Covers: the library loads and exports every entry point declared in
+include/ttsh3d.h; input.dat I/O as GetInput (TTSH3D.cu:696-752) and
+run_cuda.pl:53-87 define it, including the -8848 sentinel; parameter
+validation; the seeded initial conditions; and the conf_<m>.dat format of
+ExpoConf (TTSH3D.cu:966-1013) read back the way readDraw.py:205-215 reads it.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pandas as pd
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ttsh3d.h")
+
+# run_cuda.pl:7-44 in the input.dat order of run_cuda.pl:54-86
+RUN_CUDA_DEFAULTS = [1, 16, 1, 0, 1, 1, 1, 0, 64, 64, 64, 3, 1, 0.002, 0, 1000, 10, 0.01, 0.000005,
+                     1, -1, -0.02, 4.366, -0.187, 1, -0.364, 9, 1, 1, 1, 0, 0.0001, -8848]
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ttsh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(ttsh):
+    lib = ctypes.CDLL(ttsh.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header(ttsh):
+    ttsh.lib()
+    bound = set(ttsh.exported_symbols())
+    missing = set(declared_symbols()) - bound
+    assert not missing, missing
+
+
+def test_defaults_match_run_cuda(ttsh):
+    p = ttsh.default_params()
+    vals = [getattr(p, n) for n in ttsh.INPUT_FIELDS]
+    assert vals == pytest.approx(RUN_CUDA_DEFAULTS)
+    assert p.faithful_halo == 1
+
+
+def write_perl_style(path, values):
+    # run_cuda.pl prints one interpolated Perl number per line
+    with open(path, "w") as f:
+        for v in values:
+            f.write(f"{v}\n")
+
+
+def test_parse_run_cuda_input(ttsh, tmp_path):
+    vals = list(RUN_CUDA_DEFAULTS)
+    vals[8:11] = [128, 128, 128]
+    vals[20] = -2  # cA
+    path = tmp_path / "input.dat"
+    write_perl_style(path, vals)
+    p = ttsh.parse_input_dat(str(path))
+    assert (p.Nx, p.Ny, p.Nz) == (128, 128, 128)
+    assert p.cA == -2 and p.dtMax == 5e-6 and p.lambda0 == 9 and p.ReadSuccess == -8848
+    ttsh.validate(p)
+
+
+def test_input_roundtrip(ttsh, tmp_path):
+    p = ttsh.default_params(N=96, cA=-3.25, La=1.4, eta=-0.2, dt0=0.005, Ts=12.5)
+    path = str(tmp_path / "input.dat")
+    ttsh.write_input_dat(path, p)
+    lines = open(path).read().split("\n")
+    assert len([x for x in lines if x]) == 33
+    q = ttsh.parse_input_dat(path)
+    for n in ttsh.INPUT_FIELDS:
+        assert getattr(q, n) == getattr(p, n), n
+
+
+def test_sentinel_failure(ttsh, tmp_path):
+    vals = list(RUN_CUDA_DEFAULTS)
+    vals[-1] = 1234
+    path = tmp_path / "input.dat"
+    write_perl_style(path, vals)
+    with pytest.raises(ttsh.TTSHError) as e:
+        ttsh.parse_input_dat(str(path))
+    assert e.value.code == ttsh.ESENTINEL
+
+
+def test_short_or_missing_input(ttsh, tmp_path):
+    path = tmp_path / "input.dat"
+    write_perl_style(path, RUN_CUDA_DEFAULTS[:20])
+    with pytest.raises(ttsh.TTSHError) as e:
+        ttsh.parse_input_dat(str(path))
+    assert e.value.code == ttsh.EIO
+    with pytest.raises(ttsh.TTSHError) as e:
+        ttsh.parse_input_dat(str(tmp_path / "nope.dat"))
+    assert e.value.code == ttsh.EIO
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(TimeScheme=3), "EUNSUPPORTED"),
+    (dict(Nx=64, Ny=64, Nz=32), "EUNSUPPORTED"),
+    (dict(N=100), "EUNSUPPORTED"),
+    (dict(N=80), "EUNSUPPORTED"),
+    (dict(N=512), "EUNSUPPORTED"),
+    (dict(Nb=0), "EINVAL"),
+    (dict(dt0=0.0), "EINVAL"),
+    (dict(ReadSuccess=0), "ESENTINEL"),
+])
+def test_validation_rejects(ttsh, kw, code):
+    p = ttsh.default_params(**kw)
+    with pytest.raises(ttsh.TTSHError) as e:
+        ttsh.validate(p)
+    assert e.value.code == getattr(ttsh, code)
+
+
+@pytest.mark.parametrize("N", [32, 48, 64, 96, 128, 144, 192, 256])
+def test_validation_accepts(ttsh, N):
+    ttsh.validate(ttsh.default_params(N=N))
+
+
+def test_init_droplet_random_unit(ttsh):
+    N, R, xi = 32, 10.0, 2.0
+    phi, P, v = ttsh.init_droplet(N, R, xi, 0, seed=5)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    r = np.sqrt((x - N / 2) ** 2 + (y - N / 2) ** 2 + (z - N / 2) ** 2)
+    assert np.abs(phi - 0.5 * (1 - np.tanh((r - R) / xi))).max() < 1e-14
+    inside = phi > 0.5
+    n = np.sqrt((P ** 2).sum(0))
+    assert np.abs(n[inside] - 1).max() < 1e-12
+    assert np.abs(P[:, ~inside]).max() == 0.0
+    assert np.abs(v).max() == 0.0
+    # isotropic: the mean of each component is small, the second moments equal
+    m = P[:, inside].mean(1)
+    assert np.abs(m).max() < 0.06
+    sec = (P[:, inside] ** 2).mean(1)
+    assert np.abs(sec - 1 / 3).max() < 0.03
+    # reproducible; different seeds differ
+    phi2, P2, _ = ttsh.init_droplet(N, R, xi, 0, seed=5)
+    assert np.array_equal(P, P2)
+    _, P3, _ = ttsh.init_droplet(N, R, xi, 0, seed=6)
+    assert not np.array_equal(P, P3)
+
+
+def test_init_droplet_aster(ttsh):
+    N = 32
+    phi, P, v = ttsh.init_droplet(N, 10.0, 2.0, 1, seed=3)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    d = np.stack([x - N / 2, y - N / 2, z - N / 2])
+    r = np.sqrt((d ** 2).sum(0))
+    inside = (phi > 0.5) & (r > 0)
+    cosang = (P * d).sum(0)[inside] / r[inside]
+    assert np.median(cosang) > 0.98
+    assert np.abs(np.sqrt((P ** 2).sum(0))[inside] - 1).max() < 1e-12
+
+
+def test_init_reference(ttsh):
+    p = ttsh.default_params(N=32, init0=1e-4)
+    phi, P, v, vs, ps = ttsh.init_reference(p, seed=1)
+    N = 32
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    r = np.sqrt((x - N / 2) ** 2 + (y - N / 2) ** 2 + (z - N / 2) ** 2)
+    assert np.abs(phi - (1 - np.tanh((r - N / 4) * 2)) / 2).max() < 1e-14
+    for f in (P, v, vs, ps):
+        assert np.abs(f).max() <= 0.5e-4
+        assert np.abs(f).max() > 0.4e-4
+    assert not np.array_equal(v, vs)
+
+
+def test_conf_format(ttsh, tmp_path):
+    N = 32
+    rng = np.random.default_rng(0)
+    phi = rng.random((N, N, N)).astype(np.float32)
+    P = rng.standard_normal((3, N, N, N)).astype(np.float32)
+    v = (1e-3 * rng.standard_normal((3, N, N, N))).astype(np.float32)
+    path = str(tmp_path / "conf_0.dat")
+    ttsh.write_conf_arrays(path, phi, P, v)
+    with open(path) as f:
+        first = f.readline().rstrip("\n")
+    cols = first.split(" ")
+    assert len(cols) == 7
+    assert all(re.fullmatch(r"-?\d+\.\d{15}", c) for c in cols)
+    # readDraw.py:205-215: read_csv(sep=" ") then reshape (Ly, Lx, Lz)
+    names = ["vx", "vy", "vz", "px", "py", "pz", "phi"]
+    data = pd.read_csv(path, sep=" ", names=names)
+    assert len(data) == N ** 3
+    arrays = {"vx": v[0], "vy": v[1], "vz": v[2], "px": P[0], "py": P[1], "pz": P[2], "phi": phi}
+    for k, a in arrays.items():
+        got = data[k].to_numpy().reshape((N, N, N))          # [y, x, z]
+        want = np.transpose(a, (1, 2, 0)).astype(np.float64)  # [z, y, x] -> [y, x, z]
+        assert np.abs(got - want).max() < 1e-15 + 1e-15 * np.abs(want).max()
+
+
+def test_conf_format_expo_minus_one(ttsh, tmp_path):
+    N = 32
+    z = np.zeros((N, N, N), np.float32)
+    Z = np.zeros((3, N, N, N), np.float32)
+    path = str(tmp_path / "conf_m.dat")
+    ttsh.write_conf_arrays(path, z, Z, Z, expo_data=-1)
+    txt = open(path).read()
+    assert "\n" not in txt
+    assert txt.count(" ") == 2 * N ** 3
+
+
+def test_droplet_bytes(ttsh):
+    b = ttsh.droplet_bytes(128)
+    assert 22 * 4 * 128 ** 3 <= b < 23 * 4 * 128 ** 3
+
+
+def test_cli_rejects_bad_sentinel(tmp_path):
+    import subprocess
+    exe = os.path.join(ROOT, "bacterial-droplet_amd", "bin", "ttsh3d")
+    if not os.path.exists(exe):
+        pytest.skip("CLI not built")
+    d = tmp_path / "data_7"
+    d.mkdir()
+    vals = list(RUN_CUDA_DEFAULTS)
+    vals[-1] = 0
+    write_perl_style(d / "input.dat", vals)
+    r = subprocess.run([exe, "7"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "Error while reading the input file!" in r.stdout
