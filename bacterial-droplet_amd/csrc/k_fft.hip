@@ -1,0 +1,386 @@
+// k_fft.hip -- batched 1D FFT passes along x, y or z of a droplet's complex
+// projection buffer, with the grid size N fixed at compile time.
+//
+// These replace the reference's cufftExecZ2Z FORWARD/INVERSE on its
+// (Ny, Nx, Nz) plan (TTSH3D.cu:638, :640, :1123) and the pointwise getStream
+// type 1 and 2 kernels (-1/k2 with k2[0]=1, and the 1/N^3 scaling,
+// TTSH3D.cu:671-690), which are folded into the z pass.
+//
+// Each line of length N = R1*R2 is transformed by the four-step method
+// (Gentleman & Sande 1966; Bailey 1990): T = max(R1, R2) threads per line
+//   A: thread n1 < R1 loads x[n1 + R1 n2] (n2 < R2), does a length-R2 DFT in
+//      registers, multiplies by W_N^{n1 k2} and stores to LDS;
+//   B: thread k2 < R2 loads the R1 values of its k2, does a length-R1 DFT in
+//      registers and writes X[k2 + R2 k1].
+// Register DFTs of length 2, 3, 4 are written out; longer ones (6, 8, 12,
+// 16) are composed from them with compile-time twiddles.  Sign convention:
+// forward e^{-2 pi i jk/N}, inverse e^{+2 pi i jk/N}, both unnormalised, as
+// cufftExecZ2Z.
+#include "ttsh_dev.h"
+#include "ttsh_kernels.h"
+
+namespace ttsh {
+
+// cos and sin of 2 pi r/48 for r in [0, 12): the first quadrant of the 48th
+// roots of unity, which contain every root a DFT of length 2..16 needs.
+__device__ constexpr float kCos48[12] = {
+    1.0f, 0.99144486137381038f, 0.96592582628906831f, 0.92387953251128674f,
+    0.86602540378443865f, 0.79335334029123517f, 0.70710678118654752f, 0.60876142900872066f,
+    0.5f, 0.38268343236508977f, 0.25881904510252076f, 0.13052619222005159f};
+__device__ constexpr float kSin48[12] = {
+    0.0f, 0.13052619222005159f, 0.25881904510252076f, 0.38268343236508977f,
+    0.5f, 0.60876142900872066f, 0.70710678118654752f, 0.79335334029123517f,
+    0.86602540378443865f, 0.92387953251128674f, 0.96592582628906831f, 0.99144486137381038f};
+
+__device__ __forceinline__ float2 f2(float a, float b) { return make_float2(a, b); }
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul2(float2 a, float2 w) { return f2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x); }
+
+// a * exp(i 2 pi m/48) for a compile-time m (quarter turns are exact swaps).
+template <int M>
+__device__ __forceinline__ float2 rot48(float2 a)
+{
+    constexpr int m = ((M % 48) + 48) % 48;
+    constexpr int q = m / 12, r = m % 12;
+    float2 b = a;
+    if constexpr (r != 0) b = mul2(a, f2(kCos48[r], kSin48[r]));
+    if constexpr (q == 1) return f2(-b.y, b.x);
+    else if constexpr (q == 2) return f2(-b.x, -b.y);
+    else if constexpr (q == 3) return f2(b.y, -b.x);
+    else return b;
+}
+
+// a * W_R^{DIR*e}, W_R = exp(2 pi i/R) with DIR = -1 forward, +1 inverse.
+template <int R, int DIR, int E>
+__device__ __forceinline__ float2 twid(float2 a)
+{
+    static_assert(48 % R == 0, "register DFT length must divide 48");
+    return rot48<DIR * E * (48 / R)>(a);
+}
+
+template <int R, int DIR>
+struct Dft;
+
+template <int DIR>
+struct Dft<1, DIR> {
+    __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR>
+struct Dft<2, DIR> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        const float2 a = v[0], b = v[1];
+        v[0] = add2(a, b);
+        v[1] = sub2(a, b);
+    }
+};
+
+template <int DIR>
+struct Dft<3, DIR> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        constexpr float c3 = 0.86602540378443864676f;  // sin(2 pi/3)
+        const float2 t1 = add2(v[1], v[2]), t2 = sub2(v[1], v[2]);
+        const float2 m = f2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+        // DIR * i * sin(2pi/3) * t2
+        const float2 q = f2(-(float)DIR * c3 * t2.y, (float)DIR * c3 * t2.x);
+        v[0] = add2(v[0], t1);
+        v[1] = add2(m, q);
+        v[2] = sub2(m, q);
+    }
+};
+
+template <int DIR>
+struct Dft<4, DIR> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        const float2 s0 = add2(v[0], v[2]), d0 = sub2(v[0], v[2]);
+        const float2 s1 = add2(v[1], v[3]), d1 = sub2(v[1], v[3]);
+        const float2 j = f2(-(float)DIR * d1.y, (float)DIR * d1.x);  // DIR * i * d1
+        v[0] = add2(s0, s1);
+        v[2] = sub2(s0, s1);
+        v[1] = add2(d0, j);
+        v[3] = sub2(d0, j);
+    }
+};
+
+// Composite length R = A*B: B-point DFTs over n2 for each n1, twiddle
+// W_R^{n1 k2}, then A-point DFTs over n1; output index k2 + B k1.
+template <int A, int B, int DIR>
+struct DftComp {
+    template <int N1, int K2>
+    __device__ __forceinline__ static void tw_one(float2* t)
+    {
+        t[N1 * B + K2] = twid<A * B, DIR, N1 * K2>(t[N1 * B + K2]);
+    }
+    template <int N1, int... K2s>
+    __device__ __forceinline__ static void tw_row(float2* t, std::integer_sequence<int, K2s...>)
+    {
+        (tw_one<N1, K2s>(t), ...);
+    }
+    template <int... N1s>
+    __device__ __forceinline__ static void tw_all(float2* t, std::integer_sequence<int, N1s...>)
+    {
+        (tw_row<N1s>(t, std::make_integer_sequence<int, B>{}), ...);
+    }
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        float2 t[A * B];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) {
+            float2 u[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) u[n2] = v[n1 + A * n2];
+            Dft<B, DIR>::run(u);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) t[n1 * B + k2] = u[k2];
+        }
+        tw_all(t, std::make_integer_sequence<int, A>{});
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            float2 u[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; ++n1) u[n1] = t[n1 * B + k2];
+            Dft<A, DIR>::run(u);
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = u[k1];
+        }
+    }
+};
+
+template <int DIR> struct Dft<6, DIR> : DftComp<2, 3, DIR> {};
+template <int DIR> struct Dft<8, DIR> : DftComp<2, 4, DIR> {};
+template <int DIR> struct Dft<12, DIR> : DftComp<4, 3, DIR> {};
+template <int DIR> struct Dft<16, DIR> : DftComp<4, 4, DIR> {};
+
+// Line plan: N = R1*R2, T = max(R1,R2) threads per line, L lines per work-group.
+template <int N>
+struct Plan;
+template <> struct Plan<32>  { static constexpr int R1 = 4,  R2 = 8,  L = 32; };
+template <> struct Plan<48>  { static constexpr int R1 = 6,  R2 = 8,  L = 16; };
+template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  L = 32; };
+template <> struct Plan<96>  { static constexpr int R1 = 8,  R2 = 12, L = 16; };
+template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  L = 16; };
+template <> struct Plan<144> { static constexpr int R1 = 12, R2 = 12, L = 16; };
+template <> struct Plan<192> { static constexpr int R1 = 16, R2 = 12, L = 16; };
+template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, L = 16; };
+
+template <int N>
+struct PlanT : Plan<N> {
+    static constexpr int R1 = Plan<N>::R1, R2 = Plan<N>::R2, L = Plan<N>::L;
+    static constexpr int T = R1 > R2 ? R1 : R2;
+    static constexpr int NT = T * L;
+    static_assert(R1 * R2 == N, "plan");
+    static_assert(NT <= 256, "work-group size");
+    static_assert(N % L == 0, "lines per work-group must divide N");
+};
+
+// Pass kinds.
+enum { FFT_FWD = 0, FFT_INV = 1, FFT_POISSON = 2 };
+
+// LDS layout of the exchange buffer (L lines x R1 x R2 complex).  For the x
+// pass lanes run along a line (index k2 fastest, padded); for y/z passes
+// lanes run across lines (line fastest).
+template <int N, int AXIS>
+struct Xch {
+    using P = PlanT<N>;
+    static constexpr int R1 = P::R1, R2 = P::R2, L = P::L;
+    static constexpr int SIZE = AXIS == 0 ? L * R1 * (R2 + 1) : L * R1 * R2;
+    __device__ __forceinline__ static int at(int l, int n1, int k2)
+    {
+        if constexpr (AXIS == 0) return (l * R1 + n1) * (R2 + 1) + k2;
+        else return (n1 * R2 + k2) * L + l;
+    }
+};
+
+// -1/(k^2 N^3) at integer wavenumbers (fftfreq order), k2[0] = 1.
+__device__ __forceinline__ float poisson_factor(const Coef& co, int N, int kx, int ky, int kz)
+{
+    const int mx = kx < N / 2 ? kx : kx - N;
+    const int my = ky < N / 2 ? ky : ky - N;
+    const int mz = kz < N / 2 ? kz : kz - N;
+    const int m2 = mx * mx + my * my + mz * mz;
+    const float k2 = m2 == 0 ? 1.f : co.k2s * (float)m2;
+    return -co.inv_n3 / k2;
+}
+
+// Work item (d, a, b): droplet d; the pass's lines are the L consecutive
+// lines starting at line-block a of the plane b (x pass: rows y of plane z=b;
+// y pass: columns x of plane z=b; z pass: columns x of row y=b).
+template <int N, int AXIS, int MODE>
+__global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, const Work* __restrict__ work,
+                                              const int* __restrict__ nwork_p, int kstep)
+{
+    using P = PlanT<N>;
+    constexpr int R1 = P::R1, R2 = P::R2, L = P::L, T = P::T, NT = P::NT;
+    using X = Xch<N, AXIS>;
+    __shared__ float2 sx[X::SIZE];
+    __shared__ float2 stw[N];
+    __shared__ double red[2 * 4];
+    __shared__ double k0[2];
+
+    const int tid = threadIdx.x;
+    const bool active = tid < NT;
+    // lane -> (line l, thread-in-line t)
+    int l, t;
+    if constexpr (AXIS == 0) { l = tid / T; t = tid % T; }
+    else { t = tid / L; l = tid % L; }
+    const int nwork = *nwork_p;
+    constexpr size_t es = AXIS == 0 ? 1 : (AXIS == 1 ? (size_t)N : (size_t)N * N);
+
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const Work wk = work[w];
+        const Drop& D = drops[wk.d];
+        if (kstep >= D.nsteps) continue;
+        __syncthreads();
+        for (int i = tid; i < N; i += blockDim.x) stw[i] = D.tw[i];
+        // first element of line l
+        size_t base;
+        int lx = 0, ly = 0;  // for the Poisson factor: the line's fixed indices
+        if constexpr (AXIS == 0) {
+            ly = wk.a * L + l;
+            base = ((size_t)wk.b * N + ly) * N;
+        } else if constexpr (AXIS == 1) {
+            lx = wk.a * L + l;
+            base = (size_t)wk.b * N * N + lx;
+        } else {
+            lx = wk.a * L + l;
+            ly = wk.b;
+            base = (size_t)ly * N + lx;
+        }
+        float2* __restrict__ c = D.c;
+
+        // FP64 k=0 coefficient for the Poisson pass (work item holding kx=ky=0)
+        const bool has0 = MODE == FFT_POISSON && wk.a == 0 && wk.b == 0;
+        if (MODE == FFT_POISSON && has0) {
+            double a = 0.0, b = 0.0;
+            for (int j = tid; j < D.npart; j += blockDim.x) {
+                a += D.part[2 * j];
+                b += D.part[2 * j + 1];
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                a += __shfl_down(a, off, 64);
+                b += __shfl_down(b, off, 64);
+            }
+            if ((tid & 63) == 0) { red[2 * (tid >> 6)] = a; red[2 * (tid >> 6) + 1] = b; }
+        }
+        __syncthreads();
+        if (MODE == FFT_POISSON && has0 && tid == 0) {
+            double a = 0.0, b = 0.0;
+            for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
+            k0[0] = a;
+            k0[1] = b;
+        }
+
+        constexpr int DIR0 = MODE == FFT_INV ? 1 : -1;
+        float2 u[R1 > R2 ? R1 : R2];
+        // ---- step A: R1 threads, length-R2 DFTs -----------------------------
+        if (active && t < R1) {
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) u[n2] = c[base + (size_t)(t + R1 * n2) * es];
+            Dft<R2, DIR0>::run(u);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const float2 wv = stw[(t * k2) % N];
+                const float2 w2 = f2(wv.x, DIR0 == -1 ? wv.y : -wv.y);
+                sx[X::at(l, t, k2)] = k2 == 0 ? u[0] : mul2(u[k2], w2);
+            }
+        }
+        __syncthreads();
+        // ---- step B: R2 threads, length-R1 DFTs -----------------------------
+        if (active && t < R2) {
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) u[n1] = sx[X::at(l, n1, t)];
+            Dft<R1, DIR0>::run(u);
+        }
+        if constexpr (MODE != FFT_POISSON) {
+            if (active && t < R2) {
+#pragma unroll
+                for (int k1 = 0; k1 < R1; ++k1) c[base + (size_t)(t + R2 * k1) * es] = u[k1];
+            }
+        } else {
+            // multiply by -1/(k^2 N^3); element index kz = t + R2 k1
+            if (active && t < R2) {
+                const Coef co = D.co;
+#pragma unroll
+                for (int k1 = 0; k1 < R1; ++k1) {
+                    const int kz = t + R2 * k1;
+                    float2 s = u[k1];
+                    if (has0 && l == 0 && kz == 0) s = f2((float)k0[0], (float)k0[1]);
+                    const float m = poisson_factor(co, N, lx, ly, kz);
+                    u[k1] = f2(s.x * m, s.y * m);
+                }
+                // inverse, roles swapped: n' = t + R2 k1, thread t = n1' < R2,
+                // length-R1 DFT over k1 in registers
+                Dft<R1, 1>::run(u);
+            }
+            __syncthreads();  // every thread has read sx in step B
+            if (active && t < R2) {
+#pragma unroll
+                for (int m2 = 0; m2 < R1; ++m2) {
+                    const float2 wv = stw[(t * m2) % N];
+                    sx[X::at(l, m2, t)] = m2 == 0 ? u[0] : mul2(u[m2], f2(wv.x, -wv.y));
+                }
+            }
+            __syncthreads();
+            if (active && t < R1) {
+#pragma unroll
+                for (int n1 = 0; n1 < R2; ++n1) u[n1] = sx[X::at(l, t, n1)];
+                Dft<R2, 1>::run(u);
+#pragma unroll
+                for (int m1 = 0; m1 < R2; ++m1) c[base + (size_t)(t + R1 * m1) * es] = u[m1];
+            }
+        }
+    }
+}
+
+template <int N>
+static hipError_t launch_fft_n(int axis, int mode, const Drop* drops, const Work* work, const int* nwork_p,
+                               int kstep, int grid, hipStream_t st)
+{
+    const dim3 blk(256);
+    if (axis == 0 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 0, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else if (axis == 0 && mode == FFT_INV) hipLaunchKernelGGL((k_fft<N, 0, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else if (axis == 1 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 1, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else if (axis == 1 && mode == FFT_INV) hipLaunchKernelGGL((k_fft<N, 1, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else if (axis == 2 && mode == FFT_POISSON) hipLaunchKernelGGL((k_fft<N, 2, FFT_POISSON>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fft(int N, int axis, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep,
+                      int grid, hipStream_t st)
+{
+    if (grid <= 0) return hipSuccess;
+    switch (N) {
+    case 32: return launch_fft_n<32>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 48: return launch_fft_n<48>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 64: return launch_fft_n<64>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 96: return launch_fft_n<96>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 128: return launch_fft_n<128>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 144: return launch_fft_n<144>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 192: return launch_fft_n<192>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 256: return launch_fft_n<256>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int fft_lines_per_group(int N)
+{
+    switch (N) {
+    case 32: return PlanT<32>::L;
+    case 48: return PlanT<48>::L;
+    case 64: return PlanT<64>::L;
+    case 96: return PlanT<96>::L;
+    case 128: return PlanT<128>::L;
+    case 144: return PlanT<144>::L;
+    case 192: return PlanT<192>::L;
+    case 256: return PlanT<256>::L;
+    default: return 0;
+    }
+}
+
+}  // namespace ttsh

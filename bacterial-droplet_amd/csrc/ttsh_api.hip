@@ -1,17 +1,20 @@
 // ttsh_api.hip -- C ABI of libttsh3d: device memory, step sequencing, HIP
 // graphs and batches of droplets.
 //
-// An Engine owns a set of droplet slots on one device and one stream.  A step
-// of the engine is six launches, each covering every active droplet through a
-// work table (see ttsh_dev.h):
-//   K_rhs -> K_divx -> K_ffty(fwd) -> K_fftz -> K_ffty(inv) -> K_corrx
-// which stands in for the reference's EEuler (TTSH3D.cu:347-359: 107 kernel
+// An Engine owns a set of droplet slots on one device and one stream.  One
+// step of the engine is eight kernel kinds, each launched once per step (the
+// FFT passes once per grid size present), each covering every active
+// droplet through a work table (see ttsh_dev.h):
+//   K_rhs, K_div, FFT x fwd, FFT y fwd, FFT z (fwd, -1/k^2, inv), FFT y inv,
+//   FFT x inv, K_corr
+// standing in for the reference's EEuler (TTSH3D.cu:347-359: 107 kernel
 // launches and 4 cuFFT transforms per step).
 //
-// ttsh_sim is an engine with one slot, stepped with plain launches.  A
+// ttsh_sim is an engine with one slot stepped with plain launches.  A
 // ttsh_batch may capture K steps in a HIP graph and replay it; the kernels
-// then run with a fixed grid and read the work count from device memory, so
-// admitting or retiring droplets rewrites tables without re-capturing.
+// then run with fixed grids and read work counts from device memory, so
+// admitting or retiring droplets rewrites tables without re-capturing
+// (unless a table must grow or a grid size appears that the graph lacks).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,17 +39,38 @@ namespace ttsh {
             return fail(TTSH_EHIP, std::string(#expr " failed: ") + hipGetErrorString(e_));           \
     } while (0)
 
-enum { K_RHS = 0, K_DIV = 1, K_Y = 2, K_Z = 3, K_CORR = 4, K_NKINDS = 5 };
+// Kernel kinds of one step, in launch order.
+enum { KK_RHS = 0, KK_DIV, KK_FX, KK_FY, KK_FZ, KK_FYI, KK_FXI, KK_CORR, KK_N };
+static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_x", "K_fft_y", "K_fft_z_poisson",
+                                             "K_fft_y_inv", "K_fft_x_inv", "K_corr"};
+
+// Work tables: rhs, div/corr (shared), and per grid size the x/y-pass table
+// and the z-pass table.
+enum { T_RHS = 0, T_PROJ = 1, T_FFT0 = 2 };
+constexpr int kNTables = T_FFT0 + 2 * kNumSizes;
+static inline int t_xy(int s) { return T_FFT0 + 2 * s; }
+static inline int t_z(int s) { return T_FFT0 + 2 * s + 1; }
+
+static int size_index(int N)
+{
+    for (int s = 0; s < kNumSizes; ++s)
+        if (kSizes[s] == N) return s;
+    return -1;
+}
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-static int div_items(int N) { return (N / kDivRows) * N; }
+// Upper bound of divergence work items of one droplet (z-depth >= 1).
+static int proj_items_max(int N)
+{
+    return ((N + kProjTX - 1) / kProjTX) * ((N + kProjTY - 1) / kProjTY) * N;
+}
 
 size_t droplet_bytes(int N)
 {
     const size_t n = (size_t)N * N * N;
     return 20 * align256(n * sizeof(float)) + align256(n * sizeof(float2)) +
-           align256((size_t)div_items(N) * 2 * sizeof(double));
+           align256((size_t)proj_items_max(N) * 2 * sizeof(double));
 }
 
 static int default_rhs_variant(int N) { return (N % 32 == 0 && N >= 64) ? RHS_32x8 : RHS_16x16; }
@@ -70,7 +94,13 @@ struct Slot {
     int64_t steps = 0;      // steps since set_state / admission
     int64_t limit = -1;     // max steps, -1 = none
     double T = 0.0;         // T accumulated in double as the reference (TTSH3D.cu:342)
-    int rhs_tz = 16, corr_tz = 4;
+    int rhs_tz = 16, proj_tz = 8;
+};
+
+struct Table {
+    Work* d = nullptr;
+    int cap = 0;
+    int n = 0;
 };
 
 struct Engine {
@@ -80,7 +110,7 @@ struct Engine {
     size_t budget = 0, used = 0;
     bool use_graph = false;
     int graph_steps = 16;
-    int rhs_variant = -1;   // engine-wide K_rhs tile variant (-1: from the first droplet)
+    int rhs_variant = -1;  // engine-wide K_rhs tile variant (-1: from the first droplet)
     int cus = 256;
 
     std::vector<Slot> slots;
@@ -88,22 +118,21 @@ struct Engine {
 
     Drop* d_drops = nullptr;
     std::vector<Drop> h_drops;
-    Work* d_work[K_NKINDS] = {};
-    int work_cap[K_NKINDS] = {};
-    int nwork[K_NKINDS] = {};
-    int* d_nwork = nullptr;
-    int maxN = 0;
-    bool tables_dirty = true;
+    Table tab[kNTables];
+    int h_counts[kNTables] = {};
+    int* d_counts = nullptr;
 
     double* d_redpart = nullptr;
     double* d_redout = nullptr;
     static constexpr int kRedBlocks = 256;
 
-    // graphs, one per starting parity
+    // graphs, one per starting parity; the sizes present at capture
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    int gmaxN = 0;
-    int ggrid[K_NKINDS] = {};
+    int ggrid[kNTables] = {};
+    unsigned gsizes = 0;
     int engine_parity = 0;
+    int last_chunk = -1;
+    bool tables_dirty = true;
 
     ~Engine() { release(); }
 
@@ -111,6 +140,7 @@ struct Engine {
     {
         for (int p = 0; p < 2; ++p)
             if (gexec[p]) { (void)hipGraphExecDestroy(gexec[p]); gexec[p] = nullptr; }
+        gsizes = 0;
     }
 
     void release()
@@ -123,12 +153,12 @@ struct Engine {
         for (auto& kv : tw) (void)hipFree(kv.second);
         tw.clear();
         if (d_drops) (void)hipFree(d_drops);
-        for (int k = 0; k < K_NKINDS; ++k)
-            if (d_work[k]) { (void)hipFree(d_work[k]); d_work[k] = nullptr; }
-        if (d_nwork) (void)hipFree(d_nwork);
+        for (auto& t : tab)
+            if (t.d) { (void)hipFree(t.d); t.d = nullptr; t.cap = 0; }
+        if (d_counts) (void)hipFree(d_counts);
         if (d_redpart) (void)hipFree(d_redpart);
         if (d_redout) (void)hipFree(d_redout);
-        d_drops = nullptr; d_nwork = nullptr; d_redpart = nullptr; d_redout = nullptr;
+        d_drops = nullptr; d_counts = nullptr; d_redpart = nullptr; d_redout = nullptr;
         if (st) (void)hipStreamDestroy(st);
         st = nullptr;
     }
@@ -150,7 +180,7 @@ struct Engine {
         slots.resize(nslots);
         HIPCHK(hipMalloc(&d_drops, sizeof(Drop) * nslots));
         h_drops.resize(nslots);
-        HIPCHK(hipMalloc(&d_nwork, sizeof(int) * K_NKINDS));
+        HIPCHK(hipMalloc(&d_counts, sizeof(int) * kNTables));
         HIPCHK(hipMalloc(&d_redpart, sizeof(double) * 3 * kRedBlocks));
         HIPCHK(hipMalloc(&d_redout, sizeof(double) * 3));
         return TTSH_OK;
@@ -207,16 +237,26 @@ struct Engine {
     {
         Slot& s = slots[si];
         const int N = p.Nx;
+        if (size_index(N) < 0) return fail(TTSH_EUNSUPPORTED, "no FFT plan for N=" + std::to_string(N));
         const size_t bytes = droplet_bytes(N);
         if (budget && used + bytes > budget)
             return fail(TTSH_EBUDGET, "droplet of N=" + std::to_string(N) + " needs " + std::to_string(bytes) +
                                           " bytes; budget has " + std::to_string(budget - used) + " left");
+        int v = rhs_variant < 0 ? (p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N)) : rhs_variant;
+        if (!variant_fits(v, N)) {
+            if (p.rhs_variant >= 0) return fail(TTSH_EUNSUPPORTED, "grid smaller than the requested K_rhs tile");
+            v = RHS_16x16;  // engine-wide: every droplet fits it
+        }
         HIPCHK(hipSetDevice(device));
         void* mem = nullptr;
         hipError_t e = hipMalloc(&mem, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return fail(TTSH_ENOMEM, std::string("hipMalloc of droplet state failed: ") + hipGetErrorString(e));
+        }
+        if (v != rhs_variant) {
+            rhs_variant = v;
+            drop_graphs();
         }
         s.used = true;
         s.prm = p;
@@ -248,8 +288,7 @@ struct Engine {
         for (int q = 0; q < 2; ++q)
             for (int a = 0; a < 3; ++a) D.ps[q][a] = (float*)take(n * 4);
         D.c = (float2*)take(n * 8);
-        D.npart = div_items(N);
-        D.part = (double*)take((size_t)D.npart * 16);
+        D.part = (double*)take((size_t)proj_items_max(N) * 16);
         D.faithful = p.faithful_halo ? 1 : 0;
         make_coef(p, D.co);
         HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
@@ -258,14 +297,8 @@ struct Engine {
         s.steps = 0;
         s.limit = -1;
         s.T = p.T0;
-        s.rhs_tz = p.rhs_tz > 0 ? p.rhs_tz : 16;
-        s.corr_tz = p.corr_tz > 0 ? p.corr_tz : 4;
-        if (rhs_variant < 0) rhs_variant = p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N);
-        if (!variant_fits(rhs_variant, N)) {
-            if (p.rhs_variant >= 0) return fail(TTSH_EUNSUPPORTED, "grid smaller than the requested K_rhs tile");
-            rhs_variant = RHS_16x16;  // engine-wide: every droplet fits it
-            drop_graphs();
-        }
+        s.rhs_tz = p.rhs_tz > 0 ? std::min(p.rhs_tz, N) : 16;
+        s.proj_tz = p.corr_tz > 0 ? std::min(p.corr_tz, N) : 8;
         tables_dirty = true;
         return TTSH_OK;
     }
@@ -304,8 +337,7 @@ struct Engine {
     // Rebuild descriptor and work tables from the active slots.
     int rebuild(int steps_this_call)
     {
-        std::vector<Work> w[K_NKINDS];
-        int newmax = 0;
+        std::vector<Work> w[kNTables];
         int tx, ty;
         rhs_tile(rhs_variant < 0 ? 0 : rhs_variant, &tx, &ty);
         for (int si = 0; si < (int)slots.size(); ++si) {
@@ -316,110 +348,91 @@ struct Engine {
             h_drops[si].nsteps = (int)std::min<int64_t>(rem, (int64_t)steps_this_call);
             if (rem <= 0) continue;
             const int N = s.N;
-            newmax = std::max(newmax, N);
             for (int zb = 0; zb < N; zb += s.rhs_tz)
                 for (int by = 0; by < (N + ty - 1) / ty; ++by)
                     for (int bx = 0; bx < (N + tx - 1) / tx; ++bx)
-                        w[K_RHS].push_back(Work{si, bx, by, zb | (std::min(zb + s.rhs_tz, N) << 16)});
-            for (int z = 0; z < N; ++z)
-                for (int yb = 0; yb < N / kDivRows; ++yb) w[K_DIV].push_back(Work{si, yb, z, 0});
-            for (int z = 0; z < N; ++z)
-                for (int xb = 0; xb < N / kColTile; ++xb) w[K_Y].push_back(Work{si, xb, z, 0});
-            for (int y = 0; y < N; ++y)
-                for (int xb = 0; xb < N / kColTile; ++xb) w[K_Z].push_back(Work{si, xb, y, 0});
-            for (int zb = 0; zb < N; zb += s.corr_tz)
-                for (int yb = 0; yb < N / kCorrRows; ++yb)
-                    w[K_CORR].push_back(Work{si, yb, zb | (std::min(zb + s.corr_tz, N) << 16), 0});
+                        w[T_RHS].push_back(Work{si, bx, by, zb | (std::min(zb + s.rhs_tz, N) << 16)});
+            const int ntile = ((N + kProjTX - 1) / kProjTX) * ((N + kProjTY - 1) / kProjTY);
+            int ord = 0;
+            for (int zb = 0; zb < N; zb += s.proj_tz)
+                for (int t = 0; t < ntile; ++t)
+                    w[T_PROJ].push_back(Work{si, t, zb | (std::min(zb + s.proj_tz, N) << 16), ord++});
+            const int sidx = size_index(N);
+            const int L = fft_lines_per_group(N);
+            for (int b = 0; b < N; ++b)
+                for (int a = 0; a < N / L; ++a) {
+                    w[t_xy(sidx)].push_back(Work{si, a, b, 0});
+                    w[t_z(sidx)].push_back(Work{si, a, b, 0});
+                }
         }
-        bool realloc = false;
-        for (int k = 0; k < K_NKINDS; ++k) {
+        for (int k = 0; k < kNTables; ++k) {
             const int cnt = (int)w[k].size();
-            if (cnt > work_cap[k]) {
+            if (cnt > tab[k].cap) {
                 HIPCHK(hipStreamSynchronize(st));
-                if (d_work[k]) HIPCHK(hipFree(d_work[k]));
-                const int cap = std::max(cnt, 1024);
-                HIPCHK(hipMalloc(&d_work[k], sizeof(Work) * cap));
-                work_cap[k] = cap;
-                realloc = true;
+                if (tab[k].d) HIPCHK(hipFree(tab[k].d));
+                const int cap = std::max(cnt, 256);
+                HIPCHK(hipMalloc(&tab[k].d, sizeof(Work) * cap));
+                tab[k].cap = cap;
+                drop_graphs();  // captured launches hold the old table
             }
-            if (cnt) HIPCHK(hipMemcpyAsync(d_work[k], w[k].data(), sizeof(Work) * cnt, hipMemcpyHostToDevice, st));
-            nwork[k] = cnt;
+            if (cnt) HIPCHK(hipMemcpyAsync(tab[k].d, w[k].data(), sizeof(Work) * cnt, hipMemcpyHostToDevice, st));
+            tab[k].n = cnt;
+            h_counts[k] = cnt;
         }
-        HIPCHK(hipMemcpyAsync(d_nwork, nwork, sizeof(int) * K_NKINDS, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_counts, h_counts, sizeof(int) * kNTables, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_drops, h_drops.data(), sizeof(Drop) * slots.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));  // host tables must outlive the copies
-        if (realloc || newmax > gmaxN) drop_graphs();
-        maxN = newmax;
         tables_dirty = false;
+        last_chunk = steps_this_call;
         return TTSH_OK;
     }
 
-    // One step's six launches.  grid == nullptr means "one work-group per
-    // item".  ev (7 events, may be null) is recorded around every launch.
-    int launch_step(int parity, int kstep, const int* grid, hipEvent_t* ev = nullptr)
+    unsigned sizes_present() const
     {
-        auto g = [&](int k) { return grid ? grid[k] : nwork[k]; };
-        const int lds_n = grid ? gmaxN : maxN;
+        unsigned m = 0;
+        for (int s = 0; s < kNumSizes; ++s)
+            if (tab[t_xy(s)].n > 0) m |= 1u << s;
+        return m;
+    }
+
+    // One step's launches.  graph: use the captured fixed grids.  ev (KK_N+1
+    // events, may be null) is recorded at every kind boundary.
+    int launch_step(int parity, int kstep, bool graph, hipEvent_t* ev = nullptr)
+    {
+        auto grid = [&](int t) { return graph ? ggrid[t] : tab[t].n; };
+        auto on = [&](int s) { return graph ? ((gsizes >> s) & 1u) != 0 : tab[t_xy(s)].n > 0; };
         auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
-        HIPCHK(mark(0));
-        HIPCHK(launch_rhs(rhs_variant, d_drops, d_work[K_RHS], d_nwork + K_RHS, parity, kstep, g(K_RHS), st));
-        HIPCHK(mark(1));
-        HIPCHK(launch_divx(d_drops, d_work[K_DIV], d_nwork + K_DIV, parity, kstep, lds_n, g(K_DIV), st));
-        HIPCHK(mark(2));
-        HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 0, kstep, lds_n, g(K_Y), st));
-        HIPCHK(mark(3));
-        HIPCHK(launch_fftz(d_drops, d_work[K_Z], d_nwork + K_Z, kstep, lds_n, g(K_Z), st));
-        HIPCHK(mark(4));
-        HIPCHK(launch_ffty(d_drops, d_work[K_Y], d_nwork + K_Y, 1, kstep, lds_n, g(K_Y), st));
-        HIPCHK(mark(5));
-        HIPCHK(launch_corrx(d_drops, d_work[K_CORR], d_nwork + K_CORR, parity, kstep, lds_n, g(K_CORR), st));
-        HIPCHK(mark(6));
-        return TTSH_OK;
-    }
-
-    // n plain steps with a HIP event between consecutive kernels; ms[k] gets
-    // the summed duration of kernel k over the n steps.
-    int profile(int64_t n, double ms[6])
-    {
-        HIPCHK(hipSetDevice(device));
-        for (int k = 0; k < 6; ++k) ms[k] = 0.0;
-        constexpr int64_t kChunk = 512;  // steps recorded before the events are read
-        std::vector<hipEvent_t> ev(7 * kChunk);
-        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        int rc = TTSH_OK;
-        while (n > 0 && rc == TTSH_OK) {
-            const int64_t m = std::min(n, kChunk);
-            normalise_parity();
-            rc = rebuild((int)m);
-            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
-                rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, nullptr, &ev[7 * s]);
-            if (rc) break;
-            hipError_t e = hipEventSynchronize(ev[7 * (m - 1) + 6]);
-            if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
-            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
-                for (int k = 0; k < 6; ++k) {
-                    float t = 0.f;
-                    e = hipEventElapsedTime(&t, ev[7 * s + k], ev[7 * s + k + 1]);
-                    if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
-                    ms[k] += t;
-                }
-            advance(m);
-            n -= m;
+        const int* cnt = d_counts;
+        HIPCHK(mark(KK_RHS));
+        HIPCHK(launch_rhs(rhs_variant, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
+        HIPCHK(mark(KK_DIV));
+        HIPCHK(launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
+        const int axes[5] = {0, 1, 2, 1, 0};
+        const int modes[5] = {FFT_MODE_FWD, FFT_MODE_FWD, FFT_MODE_POISSON, FFT_MODE_INV, FFT_MODE_INV};
+        for (int f = 0; f < 5; ++f) {
+            HIPCHK(mark(KK_FX + f));
+            for (int s = 0; s < kNumSizes; ++s) {
+                if (!on(s)) continue;
+                const int t = axes[f] == 2 ? t_z(s) : t_xy(s);
+                HIPCHK(launch_fft(kSizes[s], axes[f], modes[f], d_drops, tab[t].d, cnt + t, kstep, grid(t), st));
+            }
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
-        return rc;
+        HIPCHK(mark(KK_CORR));
+        HIPCHK(launch_corr(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
+        HIPCHK(mark(KK_N));
+        return TTSH_OK;
     }
 
     int capture(int parity)
     {
         // fixed grids: enough work-groups to fill the device, never more than
-        // the tables can hold
-        gmaxN = kMaxN;
-        for (int k = 0; k < K_NKINDS; ++k) ggrid[k] = std::max(1, std::min(work_cap[k], cus * 8));
+        // a table can hold
+        for (int t = 0; t < kNTables; ++t) ggrid[t] = std::max(0, std::min(tab[t].cap, cus * 8));
+        gsizes = sizes_present();
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = TTSH_OK;
-        for (int k = 0; k < graph_steps && rc == TTSH_OK; ++k) rc = launch_step(parity ^ (k & 1), k, ggrid);
+        for (int k = 0; k < graph_steps && rc == TTSH_OK; ++k) rc = launch_step(parity ^ (k & 1), k, true);
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -438,6 +451,13 @@ struct Engine {
         for (auto& s : slots)
             if (s.used && remaining(s) > 0) ++a;
         return a;
+    }
+
+    bool limits_bind(int64_t chunk) const
+    {
+        for (auto& s : slots)
+            if (s.used && remaining(s) < chunk) return true;
+        return false;
     }
 
     void advance(int64_t k)
@@ -461,17 +481,22 @@ struct Engine {
             const bool g = graphs && use_graph && n >= graph_steps;
             const int64_t chunk = g ? graph_steps : std::min<int64_t>(n, 1 << 20);
             // per-droplet step counts of this chunk live in the descriptors
-            int rc = rebuild((int)chunk);
-            if (rc) return rc;
+            if (tables_dirty || chunk != last_chunk || limits_bind(chunk)) {
+                int rc = rebuild((int)chunk);
+                if (rc) return rc;
+            }
             if (g) {
+                if (sizes_present() & ~gsizes) drop_graphs();
                 if (!gexec[engine_parity]) {
-                    rc = capture(engine_parity);
+                    const unsigned keep = gsizes;
+                    int rc = capture(engine_parity);
                     if (rc) return rc;
+                    (void)keep;
                 }
                 HIPCHK(hipGraphLaunch(gexec[engine_parity], st));
             } else {
                 for (int64_t k = 0; k < chunk; ++k) {
-                    rc = launch_step(engine_parity ^ (int)(k & 1), (int)k, nullptr);
+                    int rc = launch_step(engine_parity ^ (int)(k & 1), (int)k, false);
                     if (rc) return rc;
                 }
             }
@@ -479,6 +504,40 @@ struct Engine {
             n -= chunk;
         }
         return TTSH_OK;
+    }
+
+    // n plain steps with a HIP event at every kernel-kind boundary; ms[k]
+    // gets the summed duration of kind k over the n steps.
+    int profile(int64_t n, double ms[KK_N])
+    {
+        HIPCHK(hipSetDevice(device));
+        for (int k = 0; k < KK_N; ++k) ms[k] = 0.0;
+        constexpr int64_t kChunk = 256;
+        constexpr int E = KK_N + 1;
+        std::vector<hipEvent_t> ev(E * kChunk);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        int rc = TTSH_OK;
+        while (n > 0 && rc == TTSH_OK) {
+            const int64_t m = std::min(n, kChunk);
+            normalise_parity();
+            if (tables_dirty || m != last_chunk || limits_bind(m)) rc = rebuild((int)m);
+            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
+                rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, false, &ev[E * s]);
+            if (rc) break;
+            hipError_t e = hipEventSynchronize(ev[E * (m - 1) + KK_N]);
+            if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
+            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
+                for (int k = 0; k < KK_N; ++k) {
+                    float t = 0.f;
+                    e = hipEventElapsedTime(&t, ev[E * s + k], ev[E * s + k + 1]);
+                    if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
+                    ms[k] += t;
+                }
+            advance(m);
+            n -= m;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc;
     }
 
     int upload_state(int si, int dtype, const void* phi, const void* P, const void* v, const void* vsp,
@@ -644,12 +703,14 @@ int ttsh_reductions(ttsh_sim* s, double out[3])
     return s->eng.reductions(0, out);
 }
 
-int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[6])
+int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[8])
 {
     if (!s || !kernel_ms) return fail(TTSH_EINVAL, "null argument");
     if (n < 0) return fail(TTSH_EINVAL, "negative step count");
     return s->eng.profile(n, kernel_ms);
 }
+
+const char* ttsh_kernel_name(int k) { return (k >= 0 && k < KK_N) ? kKindNames[k] : ""; }
 
 int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T)
 {

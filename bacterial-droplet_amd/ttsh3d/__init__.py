@@ -86,6 +86,7 @@ def lib():
         "ttsh_run": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
         "ttsh_get_clock": (C.c_int, [vp, C.POINTER(i64), dp]),
         "ttsh_profile": (C.c_int, [vp, i64, dp]),
+        "ttsh_kernel_name": (C.c_char_p, [C.c_int]),
         "ttsh_get_aux": (C.c_int, [vp, C.c_int, vp]),
         "ttsh_init_reference": (C.c_int, [P, C.c_uint64, vp, vp, vp, vp, vp]),
         "ttsh_init_droplet": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64, vp, vp, vp]),
@@ -248,13 +249,11 @@ class Simulation:
             _check(rc)
         return np.array(out[:]), rc == OK
 
-    KERNELS = ("K_rhs", "K_divx", "K_ffty_fwd", "K_fftz", "K_ffty_inv", "K_corrx")
-
     def profile(self, n: int):
-        """Run n steps with HIP events between kernels; returns {kernel: total ms}."""
-        out = (C.c_double * 6)()
+        """Run n steps with HIP events between kernel kinds; returns {kernel: total ms}."""
+        out = (C.c_double * 8)()
         _check(lib().ttsh_profile(self._h, int(n), out))
-        return dict(zip(self.KERNELS, out[:]))
+        return {lib().ttsh_kernel_name(k).decode(): out[k] for k in range(8)}
 
     def clock(self):
         s = C.c_int64(); T = C.c_double()
@@ -270,9 +269,7 @@ class Simulation:
         if which == "p":
             c = np.empty((N, N, N, 2), np.float32)
             _check(lib().ttsh_get_aux(self._h, 2, _ptr(c)))
-            z = c[..., 0].astype(np.complex128) + 1j * c[..., 1]
-            p = np.fft.ifft(z, axis=2) * N
-            return p.real, p.imag
+            return c[..., 0].astype(np.float64), c[..., 1].astype(np.float64)
         raise KeyError(which)
 
     def write_conf(self, path: str):

@@ -14,7 +14,7 @@ between consecutive kernels over the timed region give each kernel's average
 duration.  value = 128^3 * K / wall seconds / 1e6 (Mcell-steps/s).
 
 roofline: the dominant kernel's algorithmic bytes per launch (FP32 field
-traffic: K_rhs and K_corrx 56 B/cell, K_divx 32, K_ffty/K_fftz 16) divided by
+traffic: K_rhs and K_corr 56 B/cell, K_div 32, each FFT pass 16) divided by
 its average duration, against MI355X's 8 TB/s HBM peak.  traffic: HBM bytes
 per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated against a
 plain copy kernel), when bench_traffic.json beside this file holds them.
@@ -41,7 +41,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "bacterial-droplet_amd"))
 
-ALG_BYTES = {"K_rhs": 56, "K_divx": 32, "K_ffty_fwd": 16, "K_fftz": 16, "K_ffty_inv": 16, "K_corrx": 56}
+# Algorithmic FP32 field traffic per cell of each kernel kind (reads + writes):
+# K_rhs reads phi, P, v and writes phi', v*, P*; K_div reads v*, P* and writes
+# the complex s; each FFT pass reads and writes the complex buffer; K_corr
+# reads p (complex), v*, P* and writes v, P.
+ALG_BYTES = {"K_rhs": 56, "K_div": 32, "K_fft_x": 16, "K_fft_y": 16, "K_fft_z_poisson": 16,
+             "K_fft_y_inv": 16, "K_fft_x_inv": 16, "K_corr": 56}
 STEP_BYTES = 176  # algorithmic bytes per cell-step of the whole step (four-stage figure)
 HBM_PEAK_GBS = 8000.0
 

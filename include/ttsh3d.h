@@ -139,17 +139,19 @@ int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done);
 /* New (the reference only prints host CPU time, TTSH3D.cu:266,284-286): runs n
  * steps like ttsh_step with a HIP event recorded between consecutive kernels
  * on the handle's stream; kernel_ms[k] receives the summed milliseconds of
- * kernel k = K_rhs, K_divx, K_ffty (forward), K_fftz, K_ffty (inverse),
- * K_corrx.  Synchronises once per 512 steps. */
-int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[6]);
+ * kernel kind k (8 kinds, names from ttsh_kernel_name: K_rhs, K_div, the
+ * five FFT passes, K_corr).  Synchronises once per 256 steps. */
+int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[8]);
+
+/* Name of kernel kind k of ttsh_profile ("" outside 0..7). */
+const char* ttsh_kernel_name(int k);
 
 /* Steps taken since the last ttsh_set_state, and the time T. */
 int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T);
 
 /* Debug/verification access to the last step's intermediates (FP32):
- * which = 0: v* (3*N^3), 1: P* (3*N^3), 2: the complex projection buffer
- * after the inverse y/z transforms (2*N^3 interleaved re/im; an inverse DFT
- * along x gives p_v + i p_p). */
+ * which = 0: v* (3*N^3), 1: P* (3*N^3), 2: the pressures p_v + i p_p
+ * (2*N^3 interleaved re/im) that the last corrector used. */
 int ttsh_get_aux(ttsh_sim* s, int which, float* out);
 
 /* ---- initial conditions (host, FP64, seeded and reproducible) --------------- */
