@@ -50,19 +50,36 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         const bool own = gx < N && gy < N;
         float am[2] = {0.f, 0.f}, ac[2] = {0.f, 0.f}, ap[2] = {0.f, 0.f};
         double sr = 0.0, si = 0.0;
-        for (int zi = zb - 1; zi <= ze; ++zi) {
-            __syncthreads();
-            {
-                const bool wz = zi < 0 || zi >= N;
-                const int gz = wrapi(zi, N);
-                for (int i = tid; i < A1; i += NT) {
+        // the next plane is loaded into registers while the current one is used
+        constexpr int MAXP = (A1 + NT - 1) / NT;
+        float pre[MAXP][6];
+        auto fetch = [&](int zi) {
+            const bool wz = zi < 0 || zi >= N;
+            const int gz = wrapi(zi, N);
+#pragma unroll
+            for (int k = 0; k < MAXP; ++k) {
+                const int i = tid + k * NT;
+                if (i < A1) {
                     const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
                     const bool stale = wz || ex < 0 || ex >= N || ey < 0 || ey >= N;
                     const size_t g = gidx(N, wrapi(ex, N), wrapi(ey, N), gz);
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) sF[f][i] = (stale ? prv[f] : cur[f])[g];
+                    for (int f = 0; f < 6; ++f) pre[k][f] = (stale ? prv[f] : cur[f])[g];
                 }
             }
+        };
+        fetch(zb - 1);
+        for (int zi = zb - 1; zi <= ze; ++zi) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < MAXP; ++k) {
+                const int i = tid + k * NT;
+                if (i < A1) {
+#pragma unroll
+                    for (int f = 0; f < 6; ++f) sF[f][i] = pre[k][f];
+                }
+            }
+            if (zi < ze) fetch(zi + 1);
             __syncthreads();
             const int o = (ty + 1) * W1 + (tx + 1);
 #pragma unroll
@@ -135,15 +152,29 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         float2 xm = {0.f, 0.f}, xc = {0.f, 0.f}, xp = {0.f, 0.f};
         float2 ym = {0.f, 0.f}, yc = {0.f, 0.f}, yp = {0.f, 0.f};
         float2 zmv = {0.f, 0.f}, zcv = {0.f, 0.f}, zpv = {0.f, 0.f};
-        for (int zi = zb - 1; zi <= ze; ++zi) {
-            __syncthreads();
-            {
-                const int gz = wrapi(zi, N);
-                for (int i = tid; i < A1; i += NT) {
+        // the next plane is loaded into registers while the current one is used
+        constexpr int MAXP = (A1 + NT - 1) / NT;
+        float2 pre[MAXP];
+        auto fetch = [&](int zi) {
+            const int gz = wrapi(zi, N);
+#pragma unroll
+            for (int k = 0; k < MAXP; ++k) {
+                const int i = tid + k * NT;
+                if (i < A1) {
                     const int ex = wrapi(x0 - 1 + i % W1, N), ey = wrapi(y0 - 1 + i / W1, N);
-                    sp[i] = c[gidx(N, ex, ey, gz)];
+                    pre[k] = c[gidx(N, ex, ey, gz)];
                 }
             }
+        };
+        fetch(zb - 1);
+        for (int zi = zb - 1; zi <= ze; ++zi) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < MAXP; ++k) {
+                const int i = tid + k * NT;
+                if (i < A1) sp[i] = pre[k];
+            }
+            if (zi < ze) fetch(zi + 1);
             __syncthreads();
             const int o = (ty + 1) * W1 + (tx + 1);
             float2 ax, ay, bz;

@@ -211,6 +211,15 @@ struct Engine {
         co.d1 = (float)(1.0 / (2.0 * h));
         co.sw0 = (float)(4.0 / 6.0);
         co.sw1 = (float)(1.0 / 6.0);
+        co.e0 = (float)(1.0 / (2.0 * h) * (4.0 / 6.0));
+        co.e1 = (float)(1.0 / (2.0 * h) * (1.0 / 6.0));
+        co.f00 = (float)(1.0 / (2.0 * h) * (16.0 / 36.0));
+        co.f01 = (float)(1.0 / (2.0 * h) * (4.0 / 36.0));
+        co.f11 = (float)(1.0 / (2.0 * h) * (1.0 / 36.0));
+        co.lcs = (float)(6.0 / (12.0 * h * h));
+        co.lcf = (float)(-4.0 / (12.0 * h * h));
+        co.l0s = (float)(6.0 * 2.0 / (3.0 * h * h));
+        co.l0f = (float)(-14.0 / (3.0 * h * h) - 4.0 * 2.0 / (3.0 * h * h));
         co.lc = (float)(1.0 / (12.0 * h * h));
         co.lf = (float)(2.0 / (3.0 * h * h));
         co.l0 = (float)(-14.0 / (3.0 * h * h));

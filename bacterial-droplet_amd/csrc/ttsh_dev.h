@@ -35,6 +35,12 @@ struct Coef {
     float Dt;         // dt0 (Dt never changes in the reference, TTSH3D.cu:759)
     float inv_dt0;    // 1/dt0, the getStream type-0 divisor (TTSH3D.cu:665)
     float mu, cA, alpha, beta, eta, La2, lambda0, aPhi2, kPhi, gamma, A;
+    float e0, e1;     // d1*sw0, d1*sw1: weights of a transverse-smoothed difference
+    float f00, f01, f11;  // d1*sw0^2, d1*sw0*sw1, d1*sw1^2: weights of S (x) S (x) difference
+    // Laplacian from f and S_z f = (4 f + f(z-1) + f(z+1))/6 of the plane:
+    // f(z-1)+f(z+1) = 6 S_z f - 4 f, so L = lcs*sum_corners S_z f + lcf*sum_corners f
+    // + lf*sum_faces f + l0s*S_z f + l0f*f
+    float lcs, lcf, l0s, l0f;
     float k2s;        // (2 pi/(N h))^2: k2 = k2s*(mx^2+my^2+mz^2) (TTSH3D.cu:1094-1114)
     float inv_n3;     // 1/N^3, the type-2 divisor (TTSH3D.cu:687)
     int   anchoring;  // A != 0: evaluate the anchoring term

@@ -8,8 +8,10 @@
 
 namespace ttsh {
 
-// K_rhs tile variants (TX x TY cells per work-group, one cell per thread).
-enum RhsVariant { RHS_32x8 = 0, RHS_16x16 = 1, RHS_64x4 = 2, RHS_16x8 = 3, RHS_NVARIANTS = 4 };
+// K_rhs tile variants (TX x TY cells per work-group, one cell per thread, and
+// the occupancy the register allocation targets); see kRhsVars in k_rhs.hip.
+enum RhsVariant { RHS_32x8 = 0, RHS_16x16 = 1, RHS_64x4 = 2, RHS_16x8 = 3, RHS_32x8_O3 = 4, RHS_32x16 = 5,
+                  RHS_64x8 = 6, RHS_NVARIANTS = 7 };
 
 // Tile of the divergence and corrector stencils.
 constexpr int kProjTX = 32;

@@ -58,7 +58,7 @@ typedef struct ttsh_params {
     int32_t faithful_halo; /* 1 (default): div(v*), div(P*) read the previous step's
                               v*, P* across the periodic boundary, as the reference's
                               stale halo does (TTSH3D.cu:366-371,405-407,665-667) */
-    int32_t rhs_variant;   /* K_rhs tile: -1 = tuned default, else 0..3 */
+    int32_t rhs_variant;   /* K_rhs tile: -1 = tuned default, else 0..6 */
     int32_t rhs_tz;        /* K_rhs z-range per work item, 0 = tuned default */
     int32_t corr_tz;       /* corrector z-range per work item, 0 = tuned default */
 } ttsh_params;
