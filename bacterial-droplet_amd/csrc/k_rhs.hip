@@ -355,7 +355,7 @@ struct RhsVar {
     int tx, ty, minb;
 };
 static const RhsVar kRhsVars[RHS_NVARIANTS] = {
-    {32, 8, 2}, {16, 16, 2}, {64, 4, 2}, {16, 8, 4}, {32, 8, 3}, {32, 16, 1}, {64, 8, 1},
+    {32, 8, 2}, {16, 16, 2}, {64, 4, 2}, {16, 8, 2}, {32, 8, 3}, {32, 16, 1}, {64, 8, 1},
 };
 
 hipError_t launch_rhs(int variant, const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep,
@@ -365,7 +365,7 @@ hipError_t launch_rhs(int variant, const Drop* drops, const Work* work, const in
     case 0: return launch_rhs_t<32, 8, 2>(drops, work, nwork_p, parity, kstep, grid, st);
     case 1: return launch_rhs_t<16, 16, 2>(drops, work, nwork_p, parity, kstep, grid, st);
     case 2: return launch_rhs_t<64, 4, 2>(drops, work, nwork_p, parity, kstep, grid, st);
-    case 3: return launch_rhs_t<16, 8, 4>(drops, work, nwork_p, parity, kstep, grid, st);
+    case 3: return launch_rhs_t<16, 8, 2>(drops, work, nwork_p, parity, kstep, grid, st);
     case 4: return launch_rhs_t<32, 8, 3>(drops, work, nwork_p, parity, kstep, grid, st);
     case 5: return launch_rhs_t<32, 16, 1>(drops, work, nwork_p, parity, kstep, grid, st);
     case 6: return launch_rhs_t<64, 8, 1>(drops, work, nwork_p, parity, kstep, grid, st);

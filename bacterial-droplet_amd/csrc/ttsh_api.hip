@@ -251,7 +251,11 @@ struct Engine {
         if (budget && used + bytes > budget)
             return fail(TTSH_EBUDGET, "droplet of N=" + std::to_string(N) + " needs " + std::to_string(bytes) +
                                           " bytes; budget has " + std::to_string(budget - used) + " left");
-        int v = rhs_variant < 0 ? (p.rhs_variant >= 0 ? p.rhs_variant : default_rhs_variant(N)) : rhs_variant;
+        // tuned choices for N (autotuner table), unless the parameters fix them
+        int tv = -1, ttz = 0, tpz = 0;
+        if (!tune_lookup(N, &tv, &ttz, &tpz) || tv < 0 || tv >= RHS_NVARIANTS) { tv = -1; ttz = 0; tpz = 0; }
+        const int want = p.rhs_variant >= 0 ? p.rhs_variant : (tv >= 0 ? tv : default_rhs_variant(N));
+        int v = rhs_variant < 0 ? want : rhs_variant;
         if (!variant_fits(v, N)) {
             if (p.rhs_variant >= 0) return fail(TTSH_EUNSUPPORTED, "grid smaller than the requested K_rhs tile");
             v = RHS_16x16;  // engine-wide: every droplet fits it
@@ -306,8 +310,8 @@ struct Engine {
         s.steps = 0;
         s.limit = -1;
         s.T = p.T0;
-        s.rhs_tz = p.rhs_tz > 0 ? std::min(p.rhs_tz, N) : 16;
-        s.proj_tz = p.corr_tz > 0 ? std::min(p.corr_tz, N) : 8;
+        s.rhs_tz = std::min(p.rhs_tz > 0 ? p.rhs_tz : (ttz > 0 ? ttz : 16), N);
+        s.proj_tz = std::min(p.corr_tz > 0 ? p.corr_tz : (tpz > 0 ? tpz : 8), N);
         tables_dirty = true;
         return TTSH_OK;
     }

@@ -1,10 +1,16 @@
 // ttsh_host.cpp -- host-only parts of libttsh3d: parameters, input.dat I/O,
 // the conf_<m>.dat snapshot writer and the seeded initial conditions.
+#include <dlfcn.h>
+
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -54,6 +60,62 @@ bool supported_N(int N, std::string* why)
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Tuning table: per grid size N, the K_rhs tile variant, its z-depth and the
+// z-depth of the divergence/corrector stencils.  Written by the autotuner
+// (ttsh3d/autotune.py) as text lines "N rhs_variant rhs_tz proj_tz"; read from
+// $TTSH_TUNE_FILE or tuned.txt beside libttsh3d.so.
+struct TuneEntry {
+    int rhs_variant, rhs_tz, proj_tz;
+};
+static std::mutex g_tune_mu;
+static std::map<int, TuneEntry> g_tune;
+static bool g_tune_loaded = false;
+static std::string g_tune_path;
+
+static std::string default_tune_path()
+{
+    const char* env = std::getenv("TTSH_TUNE_FILE");
+    if (env && *env) return env;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&default_tune_path), &info) && info.dli_fname) {
+        std::string so(info.dli_fname);
+        const size_t slash = so.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuned.txt";
+    }
+    return "tuned.txt";
+}
+
+static int load_tune_locked(const std::string& path)
+{
+    g_tune.clear();
+    g_tune_loaded = true;
+    g_tune_path = path;
+    std::ifstream in(path);
+    if (!in.is_open()) return TTSH_EIO;
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream ss(line);
+        int N, v, rtz, ptz;
+        if (ss >> N >> v >> rtz >> ptz) g_tune[N] = TuneEntry{v, rtz, ptz};
+    }
+    return TTSH_OK;
+}
+
+bool tune_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_loaded) (void)load_tune_locked(default_tune_path());
+    auto it = g_tune.find(N);
+    if (it == g_tune.end()) return false;
+    *rhs_variant = it->second.rhs_variant;
+    *rhs_tz = it->second.rhs_tz;
+    *proj_tz = it->second.proj_tz;
+    return true;
+}
+
 // SplitMix64 (Steele, Lea & Flood, OOPSLA 2014): a counter-based stream.
 static inline uint64_t splitmix64(uint64_t x)
 {
@@ -86,6 +148,21 @@ extern "C" {
 const char* ttsh_last_error(void) { return g_last_error.c_str(); }
 
 const char* ttsh_version(void) { return "ttsh3d-mi355x 0.1"; }
+
+int ttsh_tuning_reload(const char* path)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const int rc = load_tune_locked(path ? std::string(path) : default_tune_path());
+    return rc == TTSH_OK ? TTSH_OK : fail(rc, "cannot read tuning table " + g_tune_path);
+}
+
+int ttsh_tuning_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz)
+{
+    if (!rhs_variant || !rhs_tz || !proj_tz) return fail(TTSH_EINVAL, "null argument");
+    if (!tune_lookup(N, rhs_variant, rhs_tz, proj_tz))
+        return fail(TTSH_EINVAL, "no tuning entry for N=" + std::to_string(N));
+    return TTSH_OK;
+}
 
 int ttsh_params_default(ttsh_params* p)
 {

@@ -102,6 +102,8 @@ def lib():
         "ttsh_batch_retire": (C.c_int, [vp, C.c_int]),
         "ttsh_batch_usage": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
         "ttsh_batch_synchronize": (C.c_int, [vp]),
+        "ttsh_tuning_reload": (C.c_int, [C.c_char_p]),
+        "ttsh_tuning_lookup": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "ttsh_last_error": (C.c_char_p, []),
         "ttsh_version": (C.c_char_p, []),
     }

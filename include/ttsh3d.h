@@ -226,6 +226,13 @@ const char* ttsh_last_error(void);
 /* Library version string. */
 const char* ttsh_version(void);
 
+/* New: the tuning table the autotuner writes (lines "N rhs_variant rhs_tz
+ * corr_tz"), read from $TTSH_TUNE_FILE or tuned.txt beside libttsh3d.so and
+ * applied when ttsh_params leaves rhs_variant = -1, rhs_tz = 0, corr_tz = 0.
+ * reload: re-read from path (NULL = the default location). */
+int ttsh_tuning_reload(const char* path);
+int ttsh_tuning_lookup(int N, int* rhs_variant, int* rhs_tz, int* corr_tz);
+
 #ifdef __cplusplus
 }
 #endif
