@@ -230,7 +230,9 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
     const int nwork = *nwork_p;
     constexpr size_t es = AXIS == 0 ? 1 : (AXIS == 1 ? (size_t)N : (size_t)N * N);
 
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
+        const int w = xcd_item(v, nwork);
+        if (w >= nwork) continue;
         const Work wk = work[w];
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;

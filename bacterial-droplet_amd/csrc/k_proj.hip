@@ -33,7 +33,9 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
     __shared__ double red[2 * (NT / 64)];
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int nwork = *nwork_p;
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
+        const int w = xcd_item(v, nwork);
+        if (w >= nwork) continue;
         const Work wk = work[w];
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;
@@ -136,7 +138,9 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
     __shared__ float2 sp[A1];
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int nwork = *nwork_p;
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
+        const int w = xcd_item(v, nwork);
+        if (w >= nwork) continue;
         const Work wk = work[w];
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;

@@ -97,7 +97,9 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
     const int tx = tid % TX, ty = tid / TX;
     const int nwork = *nwork_p;
 
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
+        const int w = xcd_item(v, nwork);
+        if (w >= nwork) continue;
         const Work wk = work[w];
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;

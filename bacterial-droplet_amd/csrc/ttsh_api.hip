@@ -412,7 +412,8 @@ struct Engine {
     // events, may be null) is recorded at every kind boundary.
     int launch_step(int parity, int kstep, bool graph, hipEvent_t* ev = nullptr)
     {
-        auto grid = [&](int t) { return graph ? ggrid[t] : tab[t].n; };
+        // plain launches: one work-group per visit of the XCD-blocked order
+        auto grid = [&](int t) { return graph ? ggrid[t] : ((tab[t].n + kXcds - 1) / kXcds) * kXcds; };
         auto on = [&](int s) { return graph ? ((gsizes >> s) & 1u) != 0 : tab[t_xy(s)].n > 0; };
         auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
         const int* cnt = d_counts;

@@ -71,6 +71,20 @@ struct Work {
     int d, a, b, c;
 };
 
+// Work-groups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Visit v of the grid-stride loop maps to work item
+//   (v mod 8) * chunk + v / 8,   chunk = ceil(nwork / 8),
+// so each XCD processes one contiguous, spatially ordered block of the work
+// table and tiles that share halo cache lines share an L2.  The loop runs v
+// over [0, 8*chunk) and skips items >= nwork.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_span(int nwork) { return kXcds * ((nwork + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int xcd_item(int v, int nwork)
+{
+    const int chunk = (nwork + kXcds - 1) / kXcds;
+    return (v % kXcds) * chunk + v / kXcds;
+}
+
 __device__ __forceinline__ int wrapi(int i, int N)
 {
     i += (i < 0) ? N : 0;
