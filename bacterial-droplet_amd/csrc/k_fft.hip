@@ -19,6 +19,12 @@
 #include "ttsh_dev.h"
 #include "ttsh_kernels.h"
 
+// No multiply-add contraction in this file: every pass then performs the
+// same IEEE operations whichever kernel runs a line (single-axis passes or
+// the fused x+y pass), so a droplet's result does not depend on the batch
+// it runs in.
+#pragma clang fp contract(off)
+
 namespace ttsh {
 
 // cos and sin of 2 pi r/48 for r in [0, 12): the first quadrant of the 48th
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;
         __syncthreads();
-        for (int i = tid; i < N; i += blockDim.x) stw[i] = D.tw[i];
+        for (int i = tid; i < N; i += blockDim.x) stw[i] = ld2(glb(D.tw), i);
         // first element of line l
         size_t base;
         int lx = 0, ly = 0;  // for the Poisson factor: the line's fixed indices
@@ -252,15 +258,15 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
             ly = wk.b;
             base = (size_t)ly * N + lx;
         }
-        float2* __restrict__ c = D.c;
+        const gptr<float2> c = glb(D.c);
 
         // FP64 k=0 coefficient for the Poisson pass (work item holding kx=ky=0)
         const bool has0 = MODE == FFT_POISSON && wk.a == 0 && wk.b == 0;
         if (MODE == FFT_POISSON && has0) {
             double a = 0.0, b = 0.0;
             for (int j = tid; j < D.npart; j += blockDim.x) {
-                a += D.part[2 * j];
-                b += D.part[2 * j + 1];
+                a += glb(D.part)[2 * j];
+                b += glb(D.part)[2 * j + 1];
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
         // ---- step A: R1 threads, length-R2 DFTs -----------------------------
         if (active && t < R1) {
 #pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2) u[n2] = c[base + (size_t)(t + R1 * n2) * es];
+            for (int n2 = 0; n2 < R2; ++n2) u[n2] = ld2(c, base + (size_t)(t + R1 * n2) * es);
             Dft<R2, DIR0>::run(u);
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) {
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
         if constexpr (MODE != FFT_POISSON) {
             if (active && t < R2) {
 #pragma unroll
-                for (int k1 = 0; k1 < R1; ++k1) c[base + (size_t)(t + R2 * k1) * es] = u[k1];
+                for (int k1 = 0; k1 < R1; ++k1) st2(c, base + (size_t)(t + R2 * k1) * es, u[k1]);
             }
         } else {
             // multiply by -1/(k^2 N^3); element index kz = t + R2 k1
@@ -333,9 +339,155 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
                 for (int n1 = 0; n1 < R2; ++n1) u[n1] = sx[X::at(l, t, n1)];
                 Dft<R2, 1>::run(u);
 #pragma unroll
-                for (int m1 = 0; m1 < R2; ++m1) c[base + (size_t)(t + R1 * m1) * es] = u[m1];
+                for (int m1 = 0; m1 < R2; ++m1) st2(c, base + (size_t)(t + R1 * m1) * es, u[m1]);
             }
         }
+    }
+}
+
+// ===========================================================================
+// Fused x+y pass for N <= 128: one work-group per z-plane holds the N x N
+// complex plane in LDS (row pitch N+1) and transforms all rows and all
+// columns there, so the plane crosses HBM once per direction instead of
+// twice.  Forward: x then y (e^{-i...}); inverse: y then x (e^{+i...});
+// unnormalised like the single-axis passes.
+// ===========================================================================
+template <int N>
+struct XyCfg {
+    static constexpr int NT = N >= 96 ? 1024 : 256;
+    static constexpr int P = N + 1;  // LDS row pitch in complex elements
+};
+
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// In-place DFT of the N lines of the plane along x (AX 0: line l is row y=l)
+// or y (AX 1: line l is column x=l), four-step as the single-axis passes.
+template <int N, int AX, int DIR, int NT>
+__device__ __forceinline__ void plane_dft(float2* __restrict__ pl, const float2* __restrict__ stw, int tid)
+{
+    using PL = PlanT<N>;
+    constexpr int R1 = PL::R1, R2 = PL::R2, P = XyCfg<N>::P;
+    auto at = [](int l, int j) { return AX == 0 ? l * P + j : j * P + l; };
+    // A: task (l, n1): length-R2 DFT of x[n1 + R1 n2], twiddle W_N^{n1 k2},
+    // back to the same positions.  Lanes run along a row (x lines) or across
+    // columns (y lines), contiguous in LDS either way.
+    constexpr int TA = N * R1;
+    for (int task = tid; task < TA; task += NT) {
+        const int l = AX == 0 ? task / R1 : task % N;
+        const int n1 = AX == 0 ? task % R1 : task / N;
+        float2 u[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) u[n2] = pl[at(l, n1 + R1 * n2)];
+        Dft<R2, DIR>::run(u);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const float2 wv = stw[(n1 * k2) % N];
+            const float2 w2 = f2(wv.x, DIR == -1 ? wv.y : -wv.y);
+            pl[at(l, n1 + R1 * k2)] = k2 == 0 ? u[0] : mul2(u[k2], w2);
+        }
+    }
+    __syncthreads();
+    // B: task (l, k2): length-R1 DFT over n1, result X[k2 + R2 k1]; all reads
+    // precede all writes (same plane)
+    constexpr int TB = N * R2, KB = (TB + NT - 1) / NT;
+    float2 v[KB][R1];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int task = tid + k * NT;
+        if (task < TB) {
+            const int l = AX == 0 ? task / R2 : task % N;
+            const int k2 = AX == 0 ? task % R2 : task / N;
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) v[k][n1] = pl[at(l, n1 + R1 * k2)];
+            Dft<R1, DIR>::run(v[k]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int task = tid + k * NT;
+        if (task < TB) {
+            const int l = AX == 0 ? task / R2 : task % N;
+            const int k2 = AX == 0 ? task % R2 : task / N;
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) pl[at(l, k2 + R2 * k1)] = v[k][k1];
+        }
+    }
+    __syncthreads();
+}
+
+// Work item (d, -, z): plane z of droplet d.
+template <int N, int MODE>
+__global__ __launch_bounds__(XyCfg<N>::NT) void k_fft_xy(const Drop* __restrict__ drops, const Work* __restrict__ work,
+                                                        const int* __restrict__ nwork_p, int kstep)
+{
+    constexpr int NT = XyCfg<N>::NT, P = XyCfg<N>::P;
+    constexpr int DIR = MODE == FFT_INV ? 1 : -1;
+    __shared__ float2 pl[N * P];
+    __shared__ float2 stw[N];
+    const int tid = threadIdx.x;
+    const int nwork = *nwork_p;
+    for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
+        const int w = xcd_item(v, nwork);
+        if (w >= nwork) continue;
+        const Work wk = work[w];
+        const Drop& D = drops[wk.d];
+        if (kstep >= D.nsteps) continue;
+        const gptr<fvec4> cp = (gptr<fvec4>)(glb(D.c) + (size_t)wk.b * N * N);
+        __syncthreads();  // the previous item's stores have read the plane
+        for (int i = tid; i < N; i += NT) stw[i] = ld2(glb(D.tw), i);
+        // load: two complex elements of one row per 16-byte access
+        for (int i = tid; i < N * N / 2; i += NT) {
+            const fvec4 q = cp[i];
+            const int y = (2 * i) / N, x = (2 * i) % N;
+            pl[y * P + x] = f2(q.x, q.y);
+            pl[y * P + x + 1] = f2(q.z, q.w);
+        }
+        __syncthreads();
+        if constexpr (MODE == FFT_INV) {
+            plane_dft<N, 1, DIR, NT>(pl, stw, tid);
+            plane_dft<N, 0, DIR, NT>(pl, stw, tid);
+        } else {
+            plane_dft<N, 0, DIR, NT>(pl, stw, tid);
+            plane_dft<N, 1, DIR, NT>(pl, stw, tid);
+        }
+        for (int i = tid; i < N * N / 2; i += NT) {
+            const int y = (2 * i) / N, x = (2 * i) % N;
+            const float2 a = pl[y * P + x], b = pl[y * P + x + 1];
+            fvec4 q;
+            q.x = a.x;
+            q.y = a.y;
+            q.z = b.x;
+            q.w = b.y;
+            cp[i] = q;
+        }
+    }
+}
+
+template <int N>
+static hipError_t launch_fft_xy_n(int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep,
+                                  int grid, hipStream_t st)
+{
+    const dim3 blk(XyCfg<N>::NT);
+    if (mode == FFT_FWD) hipLaunchKernelGGL((k_fft_xy<N, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else if (mode == FFT_INV) hipLaunchKernelGGL((k_fft_xy<N, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+bool fft_xy_fused(int N) { return N <= kMaxFusedXy; }
+
+hipError_t launch_fft_xy(int N, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
+                         hipStream_t st)
+{
+    if (grid <= 0) return hipSuccess;
+    switch (N) {
+    case 32: return launch_fft_xy_n<32>(mode, drops, work, nwork_p, kstep, grid, st);
+    case 48: return launch_fft_xy_n<48>(mode, drops, work, nwork_p, kstep, grid, st);
+    case 64: return launch_fft_xy_n<64>(mode, drops, work, nwork_p, kstep, grid, st);
+    case 96: return launch_fft_xy_n<96>(mode, drops, work, nwork_p, kstep, grid, st);
+    case 128: return launch_fft_xy_n<128>(mode, drops, work, nwork_p, kstep, grid, st);
+    default: return hipErrorInvalidValue;
     }
 }
 

@@ -45,9 +45,14 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         const int x0 = (wk.a % nbx) * TX, y0 = (wk.a / nbx) * TY;
         const int zb = wk.b & 0xffff, ze = wk.b >> 16;
         const int pp = D.faithful ? (parity ^ 1) : parity;
-        const float* cur[6] = {D.vs[parity][0], D.vs[parity][1], D.vs[parity][2],
-                               D.ps[parity][0], D.ps[parity][1], D.ps[parity][2]};
-        const float* prv[6] = {D.vs[pp][0], D.vs[pp][1], D.vs[pp][2], D.ps[pp][0], D.ps[pp][1], D.ps[pp][2]};
+        gptr<const float> cur[6], prv[6];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            cur[f] = glb<const float>(D.vs[parity][f]);
+            cur[3 + f] = glb<const float>(D.ps[parity][f]);
+            prv[f] = glb<const float>(D.vs[pp][f]);
+            prv[3 + f] = glb<const float>(D.ps[pp][f]);
+        }
         const int gx = x0 + tx, gy = y0 + ty;
         const bool own = gx < N && gy < N;
         float am[2] = {0.f, 0.f}, ac[2] = {0.f, 0.f}, ap[2] = {0.f, 0.f};
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             const int zo = zi - 1;
             if (zo >= zb && own) {
                 const float2 s = make_float2(am[0] * co.inv_dt0, am[1] * co.inv_dt0);
-                D.c[gidx(N, gx, gy, zo)] = s;
+                st2(glb(D.c), gidx(N, gx, gy, zo), s);
                 sr += (double)s.x;
                 si += (double)s.y;
             }
@@ -124,8 +129,8 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         if (tid == 0) {
             double a = 0.0, b = 0.0;
             for (int i = 0; i < NT / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
-            D.part[2 * wk.c] = a;
-            D.part[2 * wk.c + 1] = b;
+            glb(D.part)[2 * wk.c] = a;
+            glb(D.part)[2 * wk.c + 1] = b;
         }
     }
 }
@@ -149,7 +154,16 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         const int nbx = (N + TX - 1) / TX;
         const int x0 = (wk.a % nbx) * TX, y0 = (wk.a / nbx) * TY;
         const int zb = wk.b & 0xffff, ze = wk.b >> 16;
-        const float2* __restrict__ c = D.c;
+        const gptr<const float2> c = glb<const float2>(D.c);
+        gptr<const float> src[6];
+        gptr<float> dst[6];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            src[f] = glb<const float>(D.vs[parity][f]);
+            src[3 + f] = glb<const float>(D.ps[parity][f]);
+            dst[f] = glb(D.v[f]);
+            dst[3 + f] = glb(D.P[f]);
+        }
         const int gx = x0 + tx, gy = y0 + ty;
         const bool own = gx < N && gy < N;
         // accumulators of D_x p, D_y p, D_z p (complex) for planes zi-1, zi, zi+1
@@ -159,6 +173,8 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         // the next plane is loaded into registers while the current one is used
         constexpr int MAXP = (A1 + NT - 1) / NT;
         float2 pre[MAXP];
+        // v*, P* of the own cell at the plane that is finished next iteration
+        float sv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, nv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         auto fetch = [&](int zi) {
             const int gz = wrapi(zi, N);
 #pragma unroll
@@ -166,9 +182,14 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
                 const int i = tid + k * NT;
                 if (i < A1) {
                     const int ex = wrapi(x0 - 1 + i % W1, N), ey = wrapi(y0 - 1 + i / W1, N);
-                    pre[k] = c[gidx(N, ex, ey, gz)];
+                    pre[k] = ld2(c, gidx(N, ex, ey, gz));
                 }
             }
+            // unconditional (clamped, wrapped) so the registers need no merge
+            // copy, which would wait for the plane's loads issued just before
+            const size_t g = gidx(N, min(gx, N - 1), min(gy, N - 1), wrapi(zi - 1, N));
+#pragma unroll
+            for (int f = 0; f < 6; ++f) nv[f] = src[f][g];
         };
         fetch(zb - 1);
         for (int zi = zb - 1; zi <= ze; ++zi) {
@@ -178,6 +199,8 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
                 const int i = tid + k * NT;
                 if (i < A1) sp[i] = pre[k];
             }
+#pragma unroll
+            for (int f = 0; f < 6; ++f) sv[f] = nv[f];
             if (zi < ze) fetch(zi + 1);
             __syncthreads();
             const int o = (ty + 1) * W1 + (tx + 1);
@@ -212,12 +235,12 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
             const int zo = zi - 1;
             if (zo >= zb && own) {
                 const size_t g = gidx(N, gx, gy, zo);
-                D.v[0][g] = D.vs[parity][0][g] - co.Dt * xm.x;
-                D.v[1][g] = D.vs[parity][1][g] - co.Dt * ym.x;
-                D.v[2][g] = D.vs[parity][2][g] - co.Dt * zmv.x;
-                D.P[0][g] = D.ps[parity][0][g] - co.Dt * xm.y;
-                D.P[1][g] = D.ps[parity][1][g] - co.Dt * ym.y;
-                D.P[2][g] = D.ps[parity][2][g] - co.Dt * zmv.y;
+                dst[0][g] = sv[0] - co.Dt * xm.x;
+                dst[1][g] = sv[1] - co.Dt * ym.x;
+                dst[2][g] = sv[2] - co.Dt * zmv.x;
+                dst[3][g] = sv[3] - co.Dt * xm.y;
+                dst[4][g] = sv[4] - co.Dt * ym.y;
+                dst[5][g] = sv[5] - co.Dt * zmv.y;
             }
             xm = xc; xc = xp; xp = make_float2(0.f, 0.f);
             ym = yc; yc = yp; yp = make_float2(0.f, 0.f);

@@ -35,6 +35,8 @@
 // mu0) over the 1-cell apron into LDS; add their contribution to the D.Sigma,
 // L mu, L mu0 of the three output planes they touch (registers); finish the
 // output plane two behind the newest input plane.
+#include <type_traits>
+
 #include "ttsh_dev.h"
 #include "ttsh_kernels.h"
 
@@ -47,54 +49,96 @@ struct Der4 {
     float dx, dy, dz, l;
 };
 
-template <int W>
-__device__ __forceinline__ float lap_zp(const float* __restrict__ zf, const float* __restrict__ zs, int o,
-                                        const Coef& co)
-{
-    return co.lcs * ((zs[o - W - 1] + zs[o - W + 1]) + (zs[o + W - 1] + zs[o + W + 1]))
-         + co.lcf * ((zf[o - W - 1] + zf[o - W + 1]) + (zf[o + W - 1] + zf[o + W + 1]))
-         + co.lf * ((zf[o - 1] + zf[o + 1]) + (zf[o - W] + zf[o + W])) + co.l0s * zs[o] + co.l0f * zf[o];
-}
+// LDS tiles hold the arrays of one position together (array-of-structs) with
+// an odd stride S in floats, so lanes at consecutive positions hit distinct
+// banks.  A stencil reads three rows through three row pointers, each at the
+// x-1 neighbour of its row: every value of a row then lies within 1 KiB of
+// the row pointer, the reach of a paired LDS read's immediate offsets, so no
+// per-pair base addresses are needed.
+using lcptr = __attribute__((address_space(3))) const float*;
 
-// D_x, D_y, D_z, L at position o of a plane with row stride W from the
-// z-prepared arrays (each of the plane's size).
-template <int W>
-__device__ __forceinline__ Der4 der_zp(const float* __restrict__ zf, const float* __restrict__ zd,
-                                       const float* __restrict__ zs, int o, const Coef& co)
-{
-    Der4 r;
-    r.dx = co.e0 * (zs[o + 1] - zs[o - 1])
-         + co.e1 * ((zs[o - W + 1] - zs[o - W - 1]) + (zs[o + W + 1] - zs[o + W - 1]));
-    r.dy = co.e0 * (zs[o + W] - zs[o - W])
-         + co.e1 * ((zs[o + W - 1] - zs[o - W - 1]) + (zs[o + W + 1] - zs[o - W + 1]));
-    r.dz = co.f00 * zd[o] + co.f01 * ((zd[o - 1] + zd[o + 1]) + (zd[o - W] + zd[o + W]))
-         + co.f11 * ((zd[o - W - 1] + zd[o - W + 1]) + (zd[o + W - 1] + zd[o + W + 1]));
-    r.l = lap_zp<W>(zf, zs, o, co);
-    return r;
-}
-
-// Values of the thread's own column at one plane, kept for the output stage.
-struct Centre {
-    float G[9];  // G[3a+b] = D_a P_b
-    float mu0, phi, P[3], Dphi[3];
+struct Rows {
+    lcptr m;  // row y-1, position x-1
+    lcptr c;  // row y
+    lcptr p;  // row y+1
 };
+
+// Row pointers of position pos in a tile with row stride W and position
+// stride S.  The LDS byte addresses, array base included, are made opaque to
+// the compiler so that it does not re-derive them from one far base.
+template <int W, int S>
+__device__ __forceinline__ Rows rows_at(const float* base, int pos)
+{
+    const unsigned b0 = (unsigned)(size_t)(lcptr)base;
+    unsigned a = b0 + 4u * (unsigned)((pos - W - 1) * S);
+    unsigned b = b0 + 4u * (unsigned)((pos - 1) * S);
+    unsigned c = b0 + 4u * (unsigned)((pos + W - 1) * S);
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c));
+    return {(lcptr)(size_t)a, (lcptr)(size_t)b, (lcptr)(size_t)c};
+}
+
+// Laplacian from the f and S_z f arrays (field offset fo within a position).
+template <int S>
+__device__ __forceinline__ float lap_rows(const Rows& r, int fo, const Coef& co)
+{
+    auto F = [&](lcptr q, int dx) { return q[(dx + 1) * S + fo + ZF]; };
+    auto Z = [&](lcptr q, int dx) { return q[(dx + 1) * S + fo + ZS]; };
+    return co.lcs * ((Z(r.m, -1) + Z(r.m, 1)) + (Z(r.p, -1) + Z(r.p, 1)))
+         + co.lcf * ((F(r.m, -1) + F(r.m, 1)) + (F(r.p, -1) + F(r.p, 1)))
+         + co.lf * ((F(r.c, -1) + F(r.c, 1)) + (F(r.m, 0) + F(r.p, 0))) + co.l0s * Z(r.c, 0) + co.l0f * F(r.c, 0);
+}
+
+// D_x, D_y, D_z, L of the field at offset fo.
+template <int S>
+__device__ __forceinline__ Der4 der_rows(const Rows& r, int fo, const Coef& co)
+{
+    auto Z = [&](lcptr q, int dx) { return q[(dx + 1) * S + fo + ZS]; };
+    auto D = [&](lcptr q, int dx) { return q[(dx + 1) * S + fo + ZD]; };
+    Der4 d;
+    d.dx = co.e0 * (Z(r.c, 1) - Z(r.c, -1)) + co.e1 * ((Z(r.m, 1) - Z(r.m, -1)) + (Z(r.p, 1) - Z(r.p, -1)));
+    d.dy = co.e0 * (Z(r.p, 0) - Z(r.m, 0)) + co.e1 * ((Z(r.p, -1) - Z(r.m, -1)) + (Z(r.p, 1) - Z(r.m, 1)));
+    d.dz = co.f00 * D(r.c, 0) + co.f01 * ((D(r.c, -1) + D(r.c, 1)) + (D(r.m, 0) + D(r.p, 0)))
+         + co.f11 * ((D(r.m, -1) + D(r.m, 1)) + (D(r.p, -1) + D(r.p, 1)));
+    d.l = lap_rows<S>(r, fo, co);
+    return d;
+}
+
+// Byte-offset access to a plane: a uniform base plus a 32-bit per-lane byte
+// offset, the form of a global load or store with a scalar base address.
+__device__ __forceinline__ float ldb(gptr<const float> base, unsigned boff)
+{
+    return *(gptr<const float>)((gptr<const char>)base + boff);
+}
+__device__ __forceinline__ void stb(gptr<float> base, unsigned boff, float v)
+{
+    *(gptr<float>)((gptr<char>)base + boff) = v;
+}
+
+template <int K>
+using Ph = std::integral_constant<int, K>;
 
 template <int TX, int TY, int MINB>
 __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ drops, const Work* __restrict__ work,
                                                  const int* __restrict__ nwork_p, int parity, int kstep)
 {
     constexpr int NT = TX * TY;
-    constexpr int W2 = TX + 4, H2 = TY + 4, A2 = W2 * H2;  // inputs phi, P: 2-cell apron
+    constexpr int W2 = TX + 4, H2 = TY + 4, A2 = W2 * H2;  // phi, P, v planes: 2-cell apron layout
     constexpr int W1 = TX + 2, H1 = TY + 2, A1 = W1 * H1;  // v and intermediates: 1-cell apron
     constexpr int NRING = A1 - NT;
-    constexpr int MP2 = (A2 + NT - 1) / NT;  // input positions per thread
+    constexpr int MP2 = (A2 + NT - 1) / NT;  // phi, P positions per thread
     constexpr int MP1 = (A1 + NT - 1) / NT;  // v positions per thread
-    __shared__ float zp[4][NZ][A2];  // phi, Px, Py, Pz x (f, dz, sz)
-    __shared__ float zv[3][NZ][A1];  // vx, vy, vz x (f, dz, sz)
-    __shared__ float sM[13][A1];    // Sigma(9, [3a+b]), mu(3), mu0 of one plane
+    static_assert(NRING <= NT, "one ring position per thread at most");
+    // per position: phi, Px, Py, Pz x (f, dz, sz) | vx, vy, vz x (f, dz, sz) |
+    // Sigma(9, [3a+b]), mu(3), mu0 of one plane
+    constexpr int SP = 13, SV = 9, SS = 13;
+    __shared__ float zp[A2 * SP];
+    __shared__ float zv[A1 * SV];
+    __shared__ float sM[A1 * SS];
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
+    const int o2c = (ty + 2) * W2 + (tx + 2);  // own cell, 2-cell-apron layout
+    const int o1c = (ty + 1) * W1 + (tx + 1);  // own cell, 1-cell-apron layout
     const int nwork = *nwork_p;
 
     for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
@@ -105,93 +149,124 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
         if (kstep >= D.nsteps) continue;
         const Coef co = D.co;
         const int N = D.N;
+        const unsigned NN = (unsigned)N * (unsigned)N;
         const int x0 = wk.a * TX, y0 = wk.b * TY;
         const int zb = wk.c & 0xffff;
         const int ze = wk.c >> 16;
-        const float* __restrict__ gin[4] = {D.phi[parity], D.P[0], D.P[1], D.P[2]};
-        const float* __restrict__ gv[3] = {D.v[0], D.v[1], D.v[2]};
-        float* __restrict__ ophi = D.phi[parity ^ 1];
-        float* __restrict__ ovs[3] = {D.vs[parity][0], D.vs[parity][1], D.vs[parity][2]};
-        float* __restrict__ ops[3] = {D.ps[parity][0], D.ps[parity][1], D.ps[parity][2]};
+        const gptr<const float> gin[4] = {glb<const float>(D.phi[parity]), glb<const float>(D.P[0]),
+                                          glb<const float>(D.P[1]), glb<const float>(D.P[2])};
+        const gptr<const float> gv[3] = {glb<const float>(D.v[0]), glb<const float>(D.v[1]),
+                                         glb<const float>(D.v[2])};
+        const gptr<float> ophi = glb(D.phi[parity ^ 1]);
+        const gptr<float> ovs[3] = {glb(D.vs[parity][0]), glb(D.vs[parity][1]), glb(D.vs[parity][2])};
+        const gptr<float> ops[3] = {glb(D.ps[parity][0]), glb(D.ps[parity][1]), glb(D.ps[parity][2])};
 
         const int gx = x0 + tx, gy = y0 + ty;
         const bool own = (gx < N) && (gy < N);
+        const unsigned gown = 4u * ((unsigned)gy * (unsigned)N + (unsigned)gx);
 
-        // xy offsets (plane index) of this thread's input and v positions
-        size_t off2[MP2], off1[MP1];
+        // in-plane byte offsets of this thread's phi/P and v positions
+        unsigned off2[MP2], off1[MP1];
 #pragma unroll
         for (int k = 0; k < MP2; ++k) {
             const int i = min(tid + k * NT, A2 - 1);
-            off2[k] = (size_t)wrapi(y0 - 2 + i / W2, N) * N + wrapi(x0 - 2 + i % W2, N);
+            off2[k] = 4u * ((unsigned)wrapi(y0 - 2 + i / W2, N) * (unsigned)N + (unsigned)wrapi(x0 - 2 + i % W2, N));
         }
 #pragma unroll
         for (int k = 0; k < MP1; ++k) {
             const int i = min(tid + k * NT, A1 - 1);
-            off1[k] = (size_t)wrapi(y0 - 1 + i / W1, N) * N + wrapi(x0 - 1 + i % W1, N);
+            off1[k] = 4u * ((unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N));
         }
-        const size_t NN = (size_t)N * N;
-        // three rolling planes in registers: [0] oldest ... [2] newest
+        // three z-planes per position in registers, in slots rotating with the
+        // z-march: in phase K the slots K, K+1, K+2 (mod 3) hold planes zi-2,
+        // zi-1, zi
         float in[MP2][4][3];
         float vin[MP1][3][3];
-        auto load_in = [&](int slot, int z) {
+        auto load_plane = [&](int slot, int z, const unsigned* p2, const unsigned* p1) {
             const size_t zo = (size_t)wrapi(z, N) * NN;
 #pragma unroll
             for (int k = 0; k < MP2; ++k)
 #pragma unroll
-                for (int f = 0; f < 4; ++f) in[k][f][slot] = gin[f][zo + off2[k]];
-        };
-        auto load_v = [&](int slot, int z) {
-            const size_t zo = (size_t)wrapi(z, N) * NN;
+                for (int f = 0; f < 4; ++f) in[k][f][slot] = ldb(gin[f] + zo, p2[k]);
 #pragma unroll
             for (int k = 0; k < MP1; ++k)
 #pragma unroll
-                for (int f = 0; f < 3; ++f) vin[k][f][slot] = gv[f][zo + off1[k]];
+                for (int f = 0; f < 3; ++f) vin[k][f][slot] = ldb(gv[f] + zo, p1[k]);
         };
-        load_in(0, zb - 2);
-        load_in(1, zb - 1);
-        load_in(2, zb);
-        load_v(0, zb - 3);
-        load_v(1, zb - 2);
-        load_v(2, zb - 1);
+        load_plane(0, zb - 2, off2, off1);
+        load_plane(1, zb - 1, off2, off1);
+        load_plane(2, zb, off2, off1);
+        wait_vmem();
 
-        // accumulators for output planes zm-1 (m), zm (c), zm+1 (p)
-        float aSm[3] = {0.f, 0.f, 0.f}, aSc[3] = {0.f, 0.f, 0.f}, aSp[3] = {0.f, 0.f, 0.f};
-        float aLm[3] = {0.f, 0.f, 0.f}, aLc[3] = {0.f, 0.f, 0.f}, aLp[3] = {0.f, 0.f, 0.f};
-        float aL0m = 0.f, aL0c = 0.f, aL0p = 0.f;
-        Centre cur, nxt;
+        // D.Sigma, L mu, L mu0 accumulated per output plane, slot-rotated like
+        // the inputs: in phase K slot K is plane zm-1, K+1 is zm, K+2 is zm+1
+        float aS[3][3], aL[3][3], aL0[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            aL0[q] = 0.f;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) { aS[q][b] = 0.f; aL[q][b] = 0.f; }
+        }
+        // pointwise part of the update of the own cell, per plane:
+        //   [b]   v_b + Dt (-v.D v_b + mu L v_b + mu0 D_b phi)
+        //   [3+b] P_b + Dt (-v.D P_b - alpha phi P_b - beta |P|^2 P_b - lambda0 P.D P_b - anchoring_b)
+        //   [6]   phi + Dt (-v.D phi)
+        float cur[7], nxt[7];
 
-        for (int zi = zb; zi <= ze + 1; ++zi) {
-            // ---- 1. z-prepare plane zm = zi-1 (phi, P) and zo = zi-2 (v) ---
+        auto body = [&](auto ph, int zi) {
+            constexpr int S0 = decltype(ph)::value, S1 = (S0 + 1) % 3, S2 = (S0 + 2) % 3;
+            // LDS positions made opaque per plane: the compiler then folds each
+            // neighbour's constant offset into the LDS instruction instead of
+            // hoisting dozens of loop-invariant addresses into registers
+            // (the stencils' row pointers are opaque the same way)
+            int ti = tid;
+            asm volatile("" : "+v"(ti));
+            ti &= 0x7fff;  // known non-negative again (a no-op)
+            // likewise the in-plane global byte offsets, so that loads and
+            // stores take a scalar plane base plus this 32-bit offset
+            unsigned go = gown, q2[MP2], q1[MP1];
+            asm volatile("" : "+v"(go));
+#pragma unroll
+            for (int k = 0; k < MP2; ++k) {
+                q2[k] = off2[k];
+                asm volatile("" : "+v"(q2[k]));
+            }
+#pragma unroll
+            for (int k = 0; k < MP1; ++k) {
+                q1[k] = off1[k];
+                asm volatile("" : "+v"(q1[k]));
+            }
+            // ---- 1. z-prepare plane zm = zi-1 ------------------------------
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MP2; ++k) {
-                const int i = tid + k * NT;
+                const int i = ti + k * NT;
                 if (i < A2) {
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        const float a = in[k][f][0], b = in[k][f][1], c = in[k][f][2];
-                        const float cz = a + c;
-                        zp[f][ZF][i] = b;
-                        zp[f][ZD][i] = c - a;
-                        zp[f][ZS][i] = co.sw0 * b + co.sw1 * cz;
+                        const float a = in[k][f][S0], b = in[k][f][S1], c = in[k][f][S2];
+                        float* q = zp + i * SP + f * NZ;
+                        q[ZF] = b;
+                        q[ZD] = c - a;
+                        q[ZS] = co.sw0 * b + co.sw1 * (a + c);
                     }
                 }
             }
 #pragma unroll
             for (int k = 0; k < MP1; ++k) {
-                const int i = tid + k * NT;
+                const int i = ti + k * NT;
                 if (i < A1) {
 #pragma unroll
                     for (int f = 0; f < 3; ++f) {
-                        const float a = vin[k][f][0], b = vin[k][f][1], c = vin[k][f][2];
-                        const float cz = a + c;
-                        zv[f][ZF][i] = b;
-                        zv[f][ZD][i] = c - a;
-                        zv[f][ZS][i] = co.sw0 * b + co.sw1 * cz;
+                        const float a = vin[k][f][S0], b = vin[k][f][S1], c = vin[k][f][S2];
+                        float* q = zv + i * SV + f * NZ;
+                        q[ZF] = b;
+                        q[ZD] = c - a;
+                        q[ZS] = co.sw0 * b + co.sw1 * (a + c);
                     }
                 }
             }
-            // roll and prefetch the next planes (zi+1 inputs, zi v)
+            // roll the planes and prefetch plane zi+1 into the newest slot
 #pragma unroll
             for (int k = 0; k < MP2; ++k)
 #pragma unroll
@@ -200,64 +275,90 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             for (int k = 0; k < MP1; ++k)
 #pragma unroll
                 for (int f = 0; f < 3; ++f) { vin[k][f][0] = vin[k][f][1]; vin[k][f][1] = vin[k][f][2]; }
-            if (zi <= ze) {
-                load_in(2, zi + 1);
-                load_v(2, zi);
-            }
+            if (zi <= ze) load_plane(2, zi + 1, q2, q1);
             __syncthreads();
 
             // ---- 2. intermediates of plane zm over the 1-cell apron --------
             {
-                // the thread's own cell first (keeps its centre values)
-                const int o2 = (ty + 2) * W2 + (tx + 2);
-                const int o1 = (ty + 1) * W1 + (tx + 1);
-                const Der4 dphi = der_zp<W2>(zp[0][ZF], zp[0][ZD], zp[0][ZS], o2, co);
-                const float ph = zp[0][ZF][o2];
-                nxt.phi = ph;
-                nxt.Dphi[0] = dphi.dx; nxt.Dphi[1] = dphi.dy; nxt.Dphi[2] = dphi.dz;
+                // own cell: intermediates and the pointwise update terms
+                const Rows rp = rows_at<W2, SP>(zp, o2c);
+                const Rows rv = rows_at<W1, SV>(zv, o1c);
+                float* const mo = sM + (size_t)o1c * SS;
+                const Der4 dph = der_rows<SP>(rp, 0, co);
+                const float ph = rp.c[SP + ZF];
                 const float ph2 = ph * ph;
-                nxt.mu0 = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * dphi.l;
-                sM[12][o1] = nxt.mu0;
+                const float mu0 = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * dph.l;
+                mo[12] = mu0;
+                __builtin_amdgcn_sched_barrier(0);
+                float Pc[3], vc[3];
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const Der4 d = der_zp<W2>(zp[1 + b][ZF], zp[1 + b][ZD], zp[1 + b][ZS], o2, co);
-                    const float pb = zp[1 + b][ZF][o2];
-                    nxt.P[b] = pb;
-                    nxt.G[0 * 3 + b] = d.dx;
-                    nxt.G[1 * 3 + b] = d.dy;
-                    nxt.G[2 * 3 + b] = d.dz;
-                    sM[9 + b][o1] = ph * pb + co.La2 * d.l;
-                    sM[0 * 3 + b][o1] = ph * d.dx;
-                    sM[1 * 3 + b][o1] = ph * d.dy;
-                    sM[2 * 3 + b][o1] = ph * d.dz;
+                    Pc[b] = rp.c[SP + (1 + b) * NZ + ZF];
+                    vc[b] = rv.c[SV + b * NZ + ZF];
                 }
+                const float P2 = Pc[0] * Pc[0] + Pc[1] * Pc[1] + Pc[2] * Pc[2];
+                float an[3] = {0.f, 0.f, 0.f};
+                if (co.anchoring) {
+                    const float dP2 = dph.dx * dph.dx + dph.dy * dph.dy + dph.dz * dph.dz;
+                    const float sP = sqrtf(P2);
+                    const float dpP = dph.dx * Pc[0] + dph.dy * Pc[1] + dph.dz * Pc[2];
+                    const float Dp[3] = {dph.dx, dph.dy, dph.dz};
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) an[b] = co.A * (dP2 * P2 * Pc[b] + sP * Dp[b] * dpP);
+                }
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const Der4 d = der_rows<SP>(rp, (1 + b) * NZ, co);
+                    mo[9 + b] = ph * Pc[b] + co.La2 * d.l;
+                    mo[0 * 3 + b] = ph * d.dx;
+                    mo[1 * 3 + b] = ph * d.dy;
+                    mo[2 * 3 + b] = ph * d.dz;
+                    const float vgp = vc[0] * d.dx + vc[1] * d.dy + vc[2] * d.dz;
+                    const float pgp = Pc[0] * d.dx + Pc[1] * d.dy + Pc[2] * d.dz;
+                    const float dtp = -vgp - co.alpha * ph * Pc[b] - co.beta * P2 * Pc[b] - co.lambda0 * pgp - an[b];
+                    nxt[3 + b] = Pc[b] + co.Dt * dtp;
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                const float Dp[3] = {dph.dx, dph.dy, dph.dz};
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const Der4 d = der_rows<SV>(rv, b * NZ, co);
+                    const float vdv = vc[0] * d.dx + vc[1] * d.dy + vc[2] * d.dz;
+                    nxt[b] = vc[b] + co.Dt * (-vdv + co.mu * d.l + mu0 * Dp[b]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                nxt[6] = ph + co.Dt * (-(vc[0] * dph.dx + vc[1] * dph.dy + vc[2] * dph.dz));
             }
-            for (int r = tid; r < NRING; r += NT) {
+            if (ti < NRING) {
+                const int r = ti;
                 int lx1, ly1;
                 if (r < W1) { lx1 = r; ly1 = 0; }
                 else if (r < 2 * W1) { lx1 = r - W1; ly1 = H1 - 1; }
                 else if (r < 2 * W1 + TY) { lx1 = 0; ly1 = 1 + (r - 2 * W1); }
                 else { lx1 = W1 - 1; ly1 = 1 + (r - 2 * W1 - TY); }
-                const int o2 = (ly1 + 1) * W2 + (lx1 + 1);
-                const int o1 = ly1 * W1 + lx1;
-                const float ph = zp[0][ZF][o2];
+                const int q2 = (ly1 + 1) * W2 + (lx1 + 1);
+                const int q1 = ly1 * W1 + lx1;
+                const Rows rq = rows_at<W2, SP>(zp, q2);
+                const float ph = rq.c[SP + ZF];
                 const float ph2 = ph * ph;
-                sM[12][o1] = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * lap_zp<W2>(zp[0][ZF], zp[0][ZS], o2, co);
+                float* m = sM + q1 * SS;
+                m[12] = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * lap_rows<SP>(rq, 0, co);
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const Der4 d = der_zp<W2>(zp[1 + b][ZF], zp[1 + b][ZD], zp[1 + b][ZS], o2, co);
-                    sM[9 + b][o1] = ph * zp[1 + b][ZF][o2] + co.La2 * d.l;
-                    sM[0 * 3 + b][o1] = ph * d.dx;
-                    sM[1 * 3 + b][o1] = ph * d.dy;
-                    sM[2 * 3 + b][o1] = ph * d.dz;
+                    const Der4 d = der_rows<SP>(rq, (1 + b) * NZ, co);
+                    m[9 + b] = ph * rq.c[SP + (1 + b) * NZ + ZF] + co.La2 * d.l;
+                    m[0 * 3 + b] = ph * d.dx;
+                    m[1 * 3 + b] = ph * d.dy;
+                    m[2 * 3 + b] = ph * d.dz;
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             __syncthreads();
 
             // ---- 3. contribution of plane zm to outputs zm-1, zm, zm+1 ------
             {
-                const int o = (ty + 1) * W1 + (tx + 1);
-#define SM(f, dx, dy) sM[f][o + (dy) * W1 + (dx)]
+                const Rows rs = rows_at<W1, SS>(sM, o1c);
+#define SM(f, dx, dy) ((dy) < 0 ? rs.m : (dy) > 0 ? rs.p : rs.c)[((dx) + 1) * SS + (f)]
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
                     const int fx = 0 * 3 + b, fy = 1 * 3 + b, fz = 2 * 3 + b;
@@ -269,24 +370,25 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                     const float Bz = co.f00 * SM(fz, 0, 0)
                                    + co.f01 * ((SM(fz, -1, 0) + SM(fz, 1, 0)) + (SM(fz, 0, -1) + SM(fz, 0, 1)))
                                    + co.f11 * ((SM(fz, -1, -1) + SM(fz, 1, -1)) + (SM(fz, -1, 1) + SM(fz, 1, 1)));
-                    aSm[b] += co.sw1 * A + Bz;
-                    aSc[b] += co.sw0 * A;
-                    aSp[b] += co.sw1 * A - Bz;
+                    aS[S0][b] += co.sw1 * A + Bz;
+                    aS[S1][b] += co.sw0 * A;
+                    aS[S2][b] += co.sw1 * A - Bz;
                     const int fm = 9 + b;
                     const float cen = SM(fm, 0, 0);
                     const float E = co.lc * ((SM(fm, -1, -1) + SM(fm, 1, -1)) + (SM(fm, -1, 1) + SM(fm, 1, 1))) + co.lf * cen;
                     const float F = co.lf * ((SM(fm, -1, 0) + SM(fm, 1, 0)) + (SM(fm, 0, -1) + SM(fm, 0, 1))) + co.l0 * cen;
-                    aLm[b] += E;
-                    aLc[b] += F;
-                    aLp[b] += E;
+                    aL[S0][b] += E;
+                    aL[S1][b] += F;
+                    aL[S2][b] += E;
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 {
                     const float cen = SM(12, 0, 0);
                     const float E = co.lc * ((SM(12, -1, -1) + SM(12, 1, -1)) + (SM(12, -1, 1) + SM(12, 1, 1))) + co.lf * cen;
                     const float F = co.lf * ((SM(12, -1, 0) + SM(12, 1, 0)) + (SM(12, 0, -1) + SM(12, 0, 1))) + co.l0 * cen;
-                    aL0m += E;
-                    aL0c += F;
-                    aL0p += E;
+                    aL0[S0] += E;
+                    aL0[S1] += F;
+                    aL0[S2] += E;
                 }
 #undef SM
             }
@@ -294,50 +396,33 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             // ---- 4. finish output plane zo = zi-2 ---------------------------
             const int zo = zi - 2;
             if (zo >= zb && own) {
-                const int o = (ty + 1) * W1 + (tx + 1);
-                float vc[3], Dv[3][3], Lv[3];  // Dv[c][b] = D_c v_b
+                const size_t pz = (size_t)zo * NN;
+                stb(ophi + pz, go, cur[6] + co.Dt * (co.gamma * aL0[S0]));
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const Der4 d = der_zp<W1>(zv[b][ZF], zv[b][ZD], zv[b][ZS], o, co);
-                    vc[b] = zv[b][ZF][o];
-                    Dv[0][b] = d.dx; Dv[1][b] = d.dy; Dv[2][b] = d.dz;
-                    Lv[b] = d.l;
+                    stb(ovs[b] + pz, go, cur[b] + co.Dt * (co.cA * aS[S0][b]));
+                    stb(ops[b] + pz, go, cur[3 + b] + co.Dt * (co.eta * aL[S0][b]));
                 }
-                const float ph = cur.phi;
-                const float P2 = cur.P[0] * cur.P[0] + cur.P[1] * cur.P[1] + cur.P[2] * cur.P[2];
-                const size_t g = gidx(N, gx, gy, zo);
-                const float dtphi = -(vc[0] * cur.Dphi[0] + vc[1] * cur.Dphi[1] + vc[2] * cur.Dphi[2]) + co.gamma * aL0m;
-                ophi[g] = ph + co.Dt * dtphi;
-                float dtp[3];
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    const float dtv = -(vc[0] * Dv[0][b] + vc[1] * Dv[1][b] + vc[2] * Dv[2][b]) + co.mu * Lv[b]
-                                    + cur.mu0 * cur.Dphi[b] + co.cA * aSm[b];
-                    ovs[b][g] = vc[b] + co.Dt * dtv;
-                    const float vgp = vc[0] * cur.G[0 * 3 + b] + vc[1] * cur.G[1 * 3 + b] + vc[2] * cur.G[2 * 3 + b];
-                    const float pgp = cur.P[0] * cur.G[0 * 3 + b] + cur.P[1] * cur.G[1 * 3 + b] + cur.P[2] * cur.G[2 * 3 + b];
-                    dtp[b] = -vgp - co.alpha * ph * cur.P[b] - co.beta * P2 * cur.P[b] + co.eta * aLm[b] - co.lambda0 * pgp;
-                }
-                if (co.anchoring) {
-                    const float dP2 = cur.Dphi[0] * cur.Dphi[0] + cur.Dphi[1] * cur.Dphi[1] + cur.Dphi[2] * cur.Dphi[2];
-                    const float sP = sqrtf(P2);
-                    const float dpP = cur.Dphi[0] * cur.P[0] + cur.Dphi[1] * cur.P[1] + cur.Dphi[2] * cur.P[2];
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) dtp[b] -= co.A * (dP2 * P2 * cur.P[b] + sP * cur.Dphi[b] * dpP);
-                }
-#pragma unroll
-                for (int b = 0; b < 3; ++b) ops[b][g] = cur.P[b] + co.Dt * dtp[b];
             }
-
-            // ---- rotate accumulators and held centre values -----------------
+            // rotate the output planes: zm becomes the next zm-1
+            aL0[0] = aL0[1];
+            aL0[1] = aL0[2];
+            aL0[2] = 0.f;
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
-                aSm[b] = aSc[b]; aSc[b] = aSp[b]; aSp[b] = 0.f;
-                aLm[b] = aLc[b]; aLc[b] = aLp[b]; aLp[b] = 0.f;
+                aS[0][b] = aS[1][b];
+                aS[1][b] = aS[2][b];
+                aS[2][b] = 0.f;
+                aL[0][b] = aL[1][b];
+                aL[1][b] = aL[2][b];
+                aL[2][b] = 0.f;
             }
-            aL0m = aL0c; aL0c = aL0p; aL0p = 0.f;
-            cur = nxt;
-        }
+#pragma unroll
+            for (int j = 0; j < 7; ++j) cur[j] = nxt[j];
+        };
+
+        // z-march over zi = zb .. ze+1
+        for (int zi = zb; zi <= ze + 1; ++zi) body(Ph<0>{}, zi);
     }
 }
 

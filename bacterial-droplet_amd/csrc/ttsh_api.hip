@@ -39,17 +39,20 @@ namespace ttsh {
             return fail(TTSH_EHIP, std::string(#expr " failed: ") + hipGetErrorString(e_));           \
     } while (0)
 
-// Kernel kinds of one step, in launch order.
+// Kernel kinds of one step, in launch order.  K_fft_xy is the x pass, or the
+// fused x+y pass for sizes N <= 128 of which the batch holds at least two
+// z-planes per CU (those sizes then have no work in K_fft_y, K_fft_y_inv).
 enum { KK_RHS = 0, KK_DIV, KK_FX, KK_FY, KK_FZ, KK_FYI, KK_FXI, KK_CORR, KK_N };
-static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_x", "K_fft_y", "K_fft_z_poisson",
-                                             "K_fft_y_inv", "K_fft_x_inv", "K_corr"};
+static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_xy", "K_fft_y", "K_fft_z_poisson",
+                                             "K_fft_y_inv", "K_fft_xy_inv", "K_corr"};
 
-// Work tables: rhs, div/corr (shared), and per grid size the x/y-pass table
-// and the z-pass table.
+// Work tables: rhs, div/corr (shared), and per grid size the x/y-pass table,
+// the z-pass table and the z-plane table of the fused x+y pass.
 enum { T_RHS = 0, T_PROJ = 1, T_FFT0 = 2 };
-constexpr int kNTables = T_FFT0 + 2 * kNumSizes;
-static inline int t_xy(int s) { return T_FFT0 + 2 * s; }
-static inline int t_z(int s) { return T_FFT0 + 2 * s + 1; }
+constexpr int kNTables = T_FFT0 + 3 * kNumSizes;
+static inline int t_xy(int s) { return T_FFT0 + 3 * s; }      // line groups of x or y passes
+static inline int t_z(int s) { return T_FFT0 + 3 * s + 1; }   // line groups of the z pass
+static inline int t_pl(int s) { return T_FFT0 + 3 * s + 2; }  // z-planes of the fused x+y pass
 
 static int size_index(int N)
 {
@@ -130,6 +133,8 @@ struct Engine {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int ggrid[kNTables] = {};
     unsigned gsizes = 0;
+    // sizes that run the fused x+y pass (current tables, and at capture)
+    unsigned xy_mask = 0, gxy_mask = 0;
     int engine_parity = 0;
     int last_chunk = -1;
     bool tables_dirty = true;
@@ -141,6 +146,7 @@ struct Engine {
         for (int p = 0; p < 2; ++p)
             if (gexec[p]) { (void)hipGraphExecDestroy(gexec[p]); gexec[p] = nullptr; }
         gsizes = 0;
+        gxy_mask = 0;
     }
 
     void release()
@@ -372,11 +378,14 @@ struct Engine {
                     w[T_PROJ].push_back(Work{si, t, zb | (std::min(zb + s.proj_tz, N) << 16), ord++});
             const int sidx = size_index(N);
             const int L = fft_lines_per_group(N);
-            for (int b = 0; b < N; ++b)
+            const bool planes = fft_xy_fused(N);
+            for (int b = 0; b < N; ++b) {
+                if (planes) w[t_pl(sidx)].push_back(Work{si, 0, b, 0});
                 for (int a = 0; a < N / L; ++a) {
                     w[t_xy(sidx)].push_back(Work{si, a, b, 0});
                     w[t_z(sidx)].push_back(Work{si, a, b, 0});
                 }
+            }
         }
         for (int k = 0; k < kNTables; ++k) {
             const int cnt = (int)w[k].size();
@@ -392,6 +401,13 @@ struct Engine {
             tab[k].n = cnt;
             h_counts[k] = cnt;
         }
+        // The fused x+y pass puts one work-group on each z-plane; it pays only
+        // when a size has enough planes to give every CU two of them (a
+        // batch), otherwise the per-plane transform work leaves CUs idle and
+        // the single-axis passes are faster.
+        xy_mask = 0;
+        for (int sz = 0; sz < kNumSizes; ++sz)
+            if (tab[t_pl(sz)].n >= 2 * cus) xy_mask |= 1u << sz;
         HIPCHK(hipMemcpyAsync(d_counts, h_counts, sizeof(int) * kNTables, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_drops, h_drops.data(), sizeof(Drop) * slots.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));  // host tables must outlive the copies
@@ -404,7 +420,7 @@ struct Engine {
     {
         unsigned m = 0;
         for (int s = 0; s < kNumSizes; ++s)
-            if (tab[t_xy(s)].n > 0) m |= 1u << s;
+            if (tab[t_z(s)].n > 0) m |= 1u << s;
         return m;
     }
 
@@ -414,21 +430,30 @@ struct Engine {
     {
         // plain launches: one work-group per visit of the XCD-blocked order
         auto grid = [&](int t) { return graph ? ggrid[t] : ((tab[t].n + kXcds - 1) / kXcds) * kXcds; };
-        auto on = [&](int s) { return graph ? ((gsizes >> s) & 1u) != 0 : tab[t_xy(s)].n > 0; };
+        auto on = [&](int s) { return graph ? ((gsizes >> s) & 1u) != 0 : tab[t_z(s)].n > 0; };
         auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
         const int* cnt = d_counts;
         HIPCHK(mark(KK_RHS));
         HIPCHK(launch_rhs(rhs_variant, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
         HIPCHK(mark(KK_DIV));
         HIPCHK(launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
+        // x, y, z-Poisson, y inverse, x inverse; sizes with the fused x+y
+        // pass do x+y in the x slots and nothing in the y slots
         const int axes[5] = {0, 1, 2, 1, 0};
         const int modes[5] = {FFT_MODE_FWD, FFT_MODE_FWD, FFT_MODE_POISSON, FFT_MODE_INV, FFT_MODE_INV};
         for (int f = 0; f < 5; ++f) {
             HIPCHK(mark(KK_FX + f));
             for (int s = 0; s < kNumSizes; ++s) {
                 if (!on(s)) continue;
+                const int N = kSizes[s];
+                const bool fused = (((graph ? gxy_mask : xy_mask) >> s) & 1u) != 0;
+                if (axes[f] != 2 && fused) {
+                    if (axes[f] == 0)
+                        HIPCHK(launch_fft_xy(N, modes[f], d_drops, tab[t_pl(s)].d, cnt + t_pl(s), kstep, grid(t_pl(s)), st));
+                    continue;
+                }
                 const int t = axes[f] == 2 ? t_z(s) : t_xy(s);
-                HIPCHK(launch_fft(kSizes[s], axes[f], modes[f], d_drops, tab[t].d, cnt + t, kstep, grid(t), st));
+                HIPCHK(launch_fft(N, axes[f], modes[f], d_drops, tab[t].d, cnt + t, kstep, grid(t), st));
             }
         }
         HIPCHK(mark(KK_CORR));
@@ -443,6 +468,7 @@ struct Engine {
         // a table can hold
         for (int t = 0; t < kNTables; ++t) ggrid[t] = std::max(0, std::min(tab[t].cap, cus * 8));
         gsizes = sizes_present();
+        gxy_mask = xy_mask;
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = TTSH_OK;
@@ -500,7 +526,7 @@ struct Engine {
                 if (rc) return rc;
             }
             if (g) {
-                if (sizes_present() & ~gsizes) drop_graphs();
+                if ((sizes_present() & ~gsizes) || xy_mask != gxy_mask) drop_graphs();
                 if (!gexec[engine_parity]) {
                     const unsigned keep = gsizes;
                     int rc = capture(engine_parity);

@@ -85,6 +85,44 @@ __device__ __forceinline__ int xcd_item(int v, int nwork)
     return (v % kXcds) * chunk + v / kXcds;
 }
 
+// Global-memory views of the descriptor's pointers.  Pointers read from a
+// table in memory are generic to the compiler, which then emits flat loads and
+// stores; those count against the LDS wait counter too, so the first LDS wait
+// after a prefetch would also wait for the prefetch's HBM loads.  A cast to
+// the global address space gives global loads, tracked by the vector-memory
+// counter alone.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glb(T* p)
+{
+    return (gptr<T>)p;
+}
+
+// float2 elements of global memory go through a plain two-float vector: the
+// HIP float2 class has no assignment operators for an address-space-qualified
+// object.
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld2(gptr<const float2> p, size_t i)
+{
+    const fvec2 t = *(gptr<const fvec2>)(p + i);
+    return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void st2(gptr<float2> p, size_t i, float2 v)
+{
+    fvec2 t;
+    t.x = v.x;
+    t.y = v.y;
+    *(gptr<fvec2>)(p + i) = t;
+}
+
+// Waits for every outstanding vector-memory load (s_waitcnt vmcnt(0), the
+// export and LDS counters left at their maxima).  Issued at the end of a
+// prologue whose loads fill a loop's rolling registers, so the compiler's
+// wait analysis enters the loop with no load pending and does not fence the
+// loop's own prefetch of the next plane behind later LDS traffic.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __device__ __forceinline__ int wrapi(int i, int N)
 {
     i += (i < 0) ? N : 0;

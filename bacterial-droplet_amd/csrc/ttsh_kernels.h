@@ -40,6 +40,14 @@ hipError_t launch_fft(int N, int axis, int mode, const Drop* drops, const Work* 
                       int grid, hipStream_t st);
 // Lines per FFT work item for size N (0 if N has no plan).
 int fft_lines_per_group(int N);
+// Sizes up to this have the fused x+y pass (one work-group per z-plane,
+// the plane in LDS).
+constexpr int kMaxFusedXy = 128;
+bool fft_xy_fused(int N);
+// Fused x and y passes over every z-plane (work item (d, -, z)): mode
+// FFT_MODE_FWD transforms x then y, FFT_MODE_INV y then x.
+hipError_t launch_fft_xy(int N, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
+                         hipStream_t st);
 // FP64 reductions: per droplet sum(phi), sum|P|, sum|v|^2 -> out[3*d..].
 hipError_t launch_reduce(const Drop* drops, int ndrops, int parity, double* part, int nblk, double* out,
                          hipStream_t st);

@@ -1,17 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X TTSH3D step on one GPU.
 
-Workload (the 128^3 single-droplet configuration): periodic 128^3 box, h = 1,
+Workload (default --grid 128, the 128^3 single-droplet configuration): periodic 128^3 box, h = 1,
 droplet phi = (1 - tanh((r - 40)/2))/2 at the box centre, P a seeded random
 unit vector where phi > 0.5 (zero outside), v = 0, dt = 0.01, extensile
 activity cA = 2 x the run_cuda.pl default (-2), all other parameters at the
-run_cuda.pl defaults.  FP32 fields, FP64 reductions.
+run_cuda.pl defaults.  FP32 fields, FP64 reductions.  --grid 256 runs the 256^3
+droplet (R = 80, xi = 3, noisy aster P, run_cuda.pl defaults).
 
 A "step" is one explicit-Euler step of the reference's EEuler
 (TTSH3D.cu:347-359) on the droplet, device-resident.  W warm-up steps, then
 exactly K timed steps between two synchronisations; HIP events recorded
 between consecutive kernels over the timed region give each kernel's average
-duration.  value = 128^3 * K / wall seconds / 1e6 (Mcell-steps/s).
+duration.  value = N^3 * K / wall seconds / 1e6 (Mcell-steps/s).
 
 roofline: the dominant kernel's algorithmic bytes per launch (FP32 field
 traffic: K_rhs and K_corr 56 B/cell, K_div 32, each FFT pass 16) divided by
@@ -43,10 +44,11 @@ sys.path.insert(0, os.path.join(ROOT, "bacterial-droplet_amd"))
 
 # Algorithmic FP32 field traffic per cell of each kernel kind (reads + writes):
 # K_rhs reads phi, P, v and writes phi', v*, P*; K_div reads v*, P* and writes
-# the complex s; each FFT pass reads and writes the complex buffer; K_corr
-# reads p (complex), v*, P* and writes v, P.
-ALG_BYTES = {"K_rhs": 56, "K_div": 32, "K_fft_x": 16, "K_fft_y": 16, "K_fft_z_poisson": 16,
-             "K_fft_y_inv": 16, "K_fft_x_inv": 16, "K_corr": 56}
+# the complex s; each FFT pass reads and writes the complex buffer (K_fft_xy
+# and K_fft_xy_inv are the x passes, or in large batches the fused x+y
+# passes); K_corr reads p (complex), v*, P* and writes v, P.
+ALG_BYTES = {"K_rhs": 56, "K_div": 32, "K_fft_xy": 16, "K_fft_y": 16, "K_fft_z_poisson": 16,
+             "K_fft_y_inv": 16, "K_fft_xy_inv": 16, "K_corr": 56}
 STEP_BYTES = 176  # algorithmic bytes per cell-step of the whole step (four-stage figure)
 HBM_PEAK_GBS = 8000.0
 
@@ -55,10 +57,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload_128(ttsh, seed=1002):
-    N = 128
-    p = ttsh.default_params(N=N, dt0=0.01, cA=-2.0)
-    phi, P, v = ttsh.init_droplet(N, 40.0, 2.0, 0, seed)
+WORKLOADS = {
+    # 128^3 single droplet: R=40, xi=2, cA = 2x default, seeded random-unit P
+    128: dict(R=40.0, xi=2.0, pmode=0, cA=-2.0, seed=1002,
+              desc="single droplet 128^3, R=40, xi=2, cA=-2, random-unit P, dt=0.01"),
+    # 256^3 single droplet: R=80, xi=3, seeded noisy aster P, defaults otherwise
+    256: dict(R=80.0, xi=3.0, pmode=1, cA=-1.0, seed=1003,
+              desc="single droplet 256^3, R=80, xi=3, aster P (r_hat + 0.1 N(0,1)), dt=0.01"),
+}
+
+
+def workload(ttsh, N):
+    w = WORKLOADS[N]
+    p = ttsh.default_params(N=N, dt0=0.01, cA=w["cA"])
+    phi, P, v = ttsh.init_droplet(N, w["R"], w["xi"], w["pmode"], w["seed"])
     return p, phi, P, v
 
 
@@ -80,7 +92,7 @@ def cpu_baseline(p, phi, P, v, budget_s=15.0, max_steps=20):
     cells = p.Nx ** 3
     return {"value": round(cells * n / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": O.threads(),
             "kind": "port",
-            "sample": f"FP64 oracle (OpenMP), 128^3 droplet, {n} steps after 1 warm-up step, {dt:.1f} s"}
+            "sample": f"FP64 oracle (OpenMP), the same {p.Nx}^3 droplet, {n} steps after 1 warm-up step, {dt:.1f} s"}
 
 
 def batch_sweep(ttsh, steps, seed=1004, ndrop=64):
@@ -137,6 +149,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--batch-steps", type=int, default=64)
+    ap.add_argument("--grid", type=int, default=128, choices=sorted(WORKLOADS),
+                    help="single-droplet workload: 128 (default) or 256")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3 launches), the calibration "
                          "kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -158,7 +172,7 @@ def main():
         del src, dst
 
     import ttsh3d as ttsh
-    p, phi, P, v = workload_128(ttsh)
+    p, phi, P, v = workload(ttsh, args.grid)
     N = p.Nx
     cells = N ** 3
     sim = ttsh.Simulation(p, device=0)
@@ -188,7 +202,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
             "alg_bytes_per_cell": ALG_BYTES[dom]}
     out = {
-        "metric": "Mcell-steps/s at 128^3 FP32 (single droplet TTSH3D step)",
+        "metric": f"Mcell-steps/s at {N}^3 FP32 (single droplet TTSH3D step)",
         "value": round(value, 2),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -200,7 +214,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded droplet initial condition)",
-        "config": {"workload": "single droplet 128^3, R=40, xi=2, cA=-2, random-unit P, dt=0.01",
+        "config": {"workload": WORKLOADS[N]["desc"],
                    "grid": N, "dt": 0.01},
         "hbm_frac_step": round(value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4),
         "kernel_ms": {k: round(t, 5) for k, t in avg.items()},

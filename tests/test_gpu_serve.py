@@ -60,3 +60,29 @@ def test_step_limits_and_divergence_isolation(ttsh):
     ref = alone(ttsh, 32, good.cA, good.La, 5, 37)
     for a, c in zip(ref, got):
         assert np.array_equal(a, c)
+
+
+def test_large_batch_fused_xy_equals_alone(ttsh):
+    """Eight 64^3 droplets give 512 z-planes, enough for the engine to switch
+    that size to the fused x+y FFT pass; alone, each droplet runs the
+    single-axis passes.  Both perform the same per-line arithmetic, so the
+    results are bitwise equal."""
+    log("[large_batch_fused_xy_equals_alone]")
+    N, steps = 64, 12
+    cases = [(-1.0 * (1 + 0.25 * i), 1.0 + 0.1 * i, 300 + i) for i in range(8)]
+    f = lambda a: a.astype(np.float32)
+    with ttsh.Batch(max_slots=8, use_graph=True) as b:
+        slots = []
+        for cA, La, seed in cases:
+            p = ttsh.default_params(N=N, dt0=0.01, cA=cA, La=La)
+            phi, P, v = ttsh.init_droplet(N, 0.3 * N, 2.0, 0, seed)
+            slots.append(b.admit(p, f(phi), f(P), f(v), dtype=ttsh.F32))
+        b.step(steps)
+        got = [b.get_state(s) for s in slots]
+    for (cA, La, seed), g in zip(cases, got):
+        ref = alone(ttsh, N, cA, La, seed, steps)
+        for a, c in zip(ref, g):
+            if not np.array_equal(a, c):
+                d = np.abs(a.astype(np.float64) - c) / (np.abs(a).max() + 1e-30)
+                log(f"seed {seed}: max scaled diff {d.max():.3e}")
+            assert np.array_equal(a, c), f"droplet seed {seed} differs between batch and alone"
