@@ -10,15 +10,19 @@ droplet (R = 80, xi = 3, noisy aster P, run_cuda.pl defaults).
 
 A "step" is one explicit-Euler step of the reference's EEuler
 (TTSH3D.cu:347-359) on the droplet, device-resident.  W warm-up steps, then
-exactly K timed steps between two synchronisations; HIP events recorded
-between consecutive kernels over the timed region give each kernel's average
-duration.  value = N^3 * K / wall seconds / 1e6 (Mcell-steps/s).
+exactly K timed steps between two synchronisations, launched as a caller
+launches them (HIP-graph replays, no instrumentation); value = N^3 * K / wall
+seconds / 1e6 (Mcell-steps/s).  Then K more steps with HIP events recorded on
+the step's stream between consecutive kernel kinds give each kernel's average
+duration (kernel_ms; the events add launch gaps, so this run's own wall time,
+ms_per_step_instrumented, is longer).
 
 roofline: the dominant kernel's algorithmic bytes per launch (FP32 field
 traffic: K_rhs and K_corr 56 B/cell, K_div 32, each FFT pass 16) divided by
 its average duration, against MI355X's 8 TB/s HBM peak.  traffic: HBM bytes
 per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated against a
-plain copy kernel), when bench_traffic.json beside this file holds them.
+plain copy kernel), when bench_traffic.json beside this file holds them for
+the grid size (profiles/step_<N>_hbm_traffic.md records those runs).
 
 cpu_baseline: the FP64 oracle (OpenMP, all host threads) on the same initial
 state, timed for a bounded number of steps.
@@ -179,10 +183,18 @@ def main():
     sim.set_state(phi, P, v)
     sim.step(args.warmup)
     sim.synchronize()
+    # timed region: K steps as a caller runs them (HIP-graph replays of 16
+    # steps, no instrumentation)
     t0 = time.perf_counter()
-    kms = sim.profile(args.steps)  # the K timed steps, events between kernels
+    sim.step(args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
+    # per-kernel durations: K more steps with HIP events recorded on the
+    # step's stream at every kernel-kind boundary (plain launches)
+    t1 = time.perf_counter()
+    kms = sim.profile(args.steps)
+    sim.synchronize()
+    wall_prof = time.perf_counter() - t1
     red, finite = sim.reductions()
     sim.close()
     if not finite:
@@ -197,7 +209,7 @@ def main():
     tpath = os.path.join(ROOT, "bench_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get(str(N), {}).get(dom)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
             "alg_bytes_per_cell": ALG_BYTES[dom]}
@@ -218,6 +230,7 @@ def main():
                    "grid": N, "dt": 0.01},
         "hbm_frac_step": round(value * 1e6 * STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4),
         "kernel_ms": {k: round(t, 5) for k, t in avg.items()},
+        "ms_per_step_instrumented": round(wall_prof * 1e3 / args.steps, 5),
         "roofline": roof,
         "reductions_fp64": {"mass": red[0], "mean_abs_P": red[1], "kinetic_energy": red[2]},
     }
