@@ -328,6 +328,10 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 nxt[6] = ph + co.Dt * (-(vc[0] * dph.dx + vc[1] * dph.dy + vc[2] * dph.dz));
+                // materialise the carried terms here: sunk past the barrier,
+                // their inputs would stay live through stage 3
+#pragma unroll
+                for (int j = 0; j < 7; ++j) asm volatile("" : "+v"(nxt[j]));
             }
             if (ti < NRING) {
                 const int r = ti;

@@ -367,15 +367,23 @@ struct Engine {
             h_drops[si].nsteps = (int)std::min<int64_t>(rem, (int64_t)steps_this_call);
             if (rem <= 0) continue;
             const int N = s.N;
-            for (int zb = 0; zb < N; zb += s.rhs_tz)
+            // z is split into ceil(N/tz) chunks of near-equal depth
+            const int nzr = (N + s.rhs_tz - 1) / s.rhs_tz;
+            for (int k = 0; k < nzr; ++k) {
+                const int zb = k * N / nzr, ze = (k + 1) * N / nzr;
                 for (int by = 0; by < (N + ty - 1) / ty; ++by)
                     for (int bx = 0; bx < (N + tx - 1) / tx; ++bx)
-                        w[T_RHS].push_back(Work{si, bx, by, zb | (std::min(zb + s.rhs_tz, N) << 16)});
+                        w[T_RHS].push_back(Work{si, bx, by, zb | (ze << 16)});
+            }
             const int ntile = ((N + kProjTX - 1) / kProjTX) * ((N + kProjTY - 1) / kProjTY);
+            const int nzp = (N + s.proj_tz - 1) / s.proj_tz;
             int ord = 0;
-            for (int zb = 0; zb < N; zb += s.proj_tz)
-                for (int t = 0; t < ntile; ++t)
-                    w[T_PROJ].push_back(Work{si, t, zb | (std::min(zb + s.proj_tz, N) << 16), ord++});
+            for (int k = 0; k < nzp; ++k) {
+                const int zb = k * N / nzp, ze = (k + 1) * N / nzp;
+                for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), ord++});
+            }
+            // the Poisson pass sums the divergence items' FP64 partial sums
+            h_drops[si].npart = ord;
             const int sidx = size_index(N);
             const int L = fft_lines_per_group(N);
             const bool planes = fft_xy_fused(N);

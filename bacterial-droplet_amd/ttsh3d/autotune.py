@@ -36,7 +36,9 @@ def tune_size(N: int, steps: int = 30, log=print):
     base = dict(N=N, dt0=0.01, cA=-2.0)
     best_rhs, best_proj = None, None
     results = []
-    tzs = [t for t in (8, 16, 32) if t <= N]
+    # z-depths: the engine splits z into ceil(N/tz) near-equal chunks, so
+    # depths like 11 or 43 set the number of work items per CU
+    tzs = sorted({t for t in (8, 11, 16, 22, 32, 43, 64) if t <= N})
     for var, (tx, ty) in RHS_VARIANTS.items():
         if N < tx + 2 or N < ty + 2:
             continue
@@ -50,7 +52,7 @@ def tune_size(N: int, steps: int = 30, log=print):
             if best_rhs is None or k["K_rhs"] < best_rhs[2]:
                 best_rhs = (var, tz, k["K_rhs"])
             log(f"N={N} K_rhs variant {var} ({tx}x{ty}) tz {tz}: {k['K_rhs'] * 1e3:.1f} us")
-    for ptz in [t for t in (4, 8, 16, 32) if t <= N]:
+    for ptz in [t for t in (4, 6, 8, 11, 16, 32) if t <= N]:
         k = _time(default_params(rhs_variant=best_rhs[0], rhs_tz=best_rhs[1], corr_tz=ptz, **base), state, steps)
         t = k["K_div"] + k["K_corr"]
         results.append({"kind": "proj", "tz": ptz, "ms": t})

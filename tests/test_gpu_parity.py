@@ -66,6 +66,11 @@ def test_one_step_stages(ttsh, oracle_mod, N):
         log("stages", {k: f"{x:.2e}" for k, x in e.items()})
         for k, x in e.items():
             assert x <= (1e-4 if k == "p_v" else 2e-5), f"{k}: {x}"
+        # the k = 0 mode of p (k2[0] = 1): the mean, from the FP64 sum of s
+        for name, g in (("p_v", pv), ("p_p", pp)):
+            ref = o.aux(name)
+            log(f"{name} mean gpu {g.mean():.6e} oracle {ref.mean():.6e} rms {np.sqrt((ref ** 2).mean()):.3e}")
+            assert abs(g.mean() - ref.mean()) <= 1e-5 * np.sqrt((ref ** 2).mean()) + 1e-12 * abs(ref.mean()), name
         compare_state(sim, o, 2e-5, f"N={N} 1 step")
         steps, T = sim.clock()
         assert steps == 1 and T == pytest.approx(0.01)
