@@ -179,7 +179,8 @@ struct PlanT : Plan<N> {
     static constexpr int T = R1 > R2 ? R1 : R2;
     static constexpr int NT = T * L;
     static_assert(R1 * R2 == N, "plan");
-    static_assert(NT <= 256, "work-group size");
+    static constexpr int NB = (NT + 63) / 64 * 64;  // work-group size
+    static_assert(NT <= 1024, "work-group size");
     static_assert(N % L == 0, "lines per work-group must divide N");
 };
 
@@ -216,7 +217,7 @@ __device__ __forceinline__ float poisson_factor(const Coef& co, int N, int kx, i
 // lines starting at line-block a of the plane b (x pass: rows y of plane z=b;
 // y pass: columns x of plane z=b; z pass: columns x of row y=b).
 template <int N, int AXIS, int MODE>
-__global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, const Work* __restrict__ work,
+__global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ drops, const Work* __restrict__ work,
                                               const int* __restrict__ nwork_p, int kstep)
 {
     using P = PlanT<N>;
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(256) void k_fft(const Drop* __restrict__ drops, con
     using X = Xch<N, AXIS>;
     __shared__ float2 sx[X::SIZE];
     __shared__ float2 stw[N];
-    __shared__ double red[2 * 4];
+    __shared__ double red[2 * (P::NB / 64)];
     __shared__ double k0[2];
 
     const int tid = threadIdx.x;
@@ -495,7 +496,7 @@ template <int N>
 static hipError_t launch_fft_n(int axis, int mode, const Drop* drops, const Work* work, const int* nwork_p,
                                int kstep, int grid, hipStream_t st)
 {
-    const dim3 blk(256);
+    const dim3 blk(PlanT<N>::NB);
     if (axis == 0 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 0, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
     else if (axis == 0 && mode == FFT_INV) hipLaunchKernelGGL((k_fft<N, 0, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
     else if (axis == 1 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 1, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);

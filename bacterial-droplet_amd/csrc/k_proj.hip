@@ -24,6 +24,7 @@
 
 namespace ttsh {
 
+
 template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops, const Work* __restrict__ work,
                                                  const int* __restrict__ nwork_p, int parity, int kstep)
@@ -55,38 +56,62 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         }
         const int gx = x0 + tx, gy = y0 + ty;
         const bool own = gx < N && gy < N;
+        const size_t NN = (size_t)N * N;
         float am[2] = {0.f, 0.f}, ac[2] = {0.f, 0.f}, ap[2] = {0.f, 0.f};
         double sr = 0.0, si = 0.0;
-        // the next plane is loaded into registers while the current one is used
+        // Per field, the current and the previous-step buffer are addressed
+        // from the lower of the two (uniform base lo[f]) with a per-lane byte
+        // offset that adds the distance dh[f] to reach the higher one.  Every
+        // load is then one scalar-base load; the buffers of a droplet share
+        // one allocation, so the distance fits 32 bits.
+        gptr<const float> lo[6];
+        unsigned dh[6];
+        bool prv_hi[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            prv_hi[f] = prv[f] > cur[f];
+            lo[f] = prv_hi[f] ? cur[f] : prv[f];
+            dh[f] = 4u * (unsigned)(prv_hi[f] ? prv[f] - cur[f] : cur[f] - prv[f]);
+        }
+        // in-plane byte offsets of this thread's apron positions (clamped, so
+        // every thread loads every slot), within a plane taken from the
+        // previous step where the position lies outside the box in x or y
         constexpr int MAXP = (A1 + NT - 1) / NT;
-        float pre[MAXP][6];
-        auto fetch = [&](int zi) {
+        unsigned off[MAXP], oxy[MAXP][6];
+#pragma unroll
+        for (int k = 0; k < MAXP; ++k) {
+            const int i = min(tid + k * NT, A1 - 1);
+            const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
+            const bool sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
+            off[k] = 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
+#pragma unroll
+            for (int f = 0; f < 6; ++f) oxy[k][f] = off[k] + ((sxy == prv_hi[f]) ? dh[f] : 0u);
+        }
+        // planes are loaded two ahead, into two register slots; a plane
+        // outside the box (wrapped in z) is read wholly from the previous step
+        float pre[2][MAXP][6];
+        auto fetch = [&](int slot, int zi) {
             const bool wz = zi < 0 || zi >= N;
-            const int gz = wrapi(zi, N);
+            const size_t zo = (size_t)wrapi(zi, N) * NN;
 #pragma unroll
-            for (int k = 0; k < MAXP; ++k) {
-                const int i = tid + k * NT;
-                if (i < A1) {
-                    const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
-                    const bool stale = wz || ex < 0 || ex >= N || ey < 0 || ey >= N;
-                    const size_t g = gidx(N, wrapi(ex, N), wrapi(ey, N), gz);
+            for (int f = 0; f < 6; ++f) {
+                const unsigned adj = prv_hi[f] ? dh[f] : 0u;  // whole plane stale
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) pre[k][f] = (stale ? prv[f] : cur[f])[g];
-                }
+                for (int k = 0; k < MAXP; ++k) pre[slot][k][f] = ldb(lo[f] + zo, wz ? off[k] + adj : oxy[k][f]);
             }
         };
-        fetch(zb - 1);
-        for (int zi = zb - 1; zi <= ze; ++zi) {
+        auto body = [&](auto ph, int zi) {
+            constexpr int S = decltype(ph)::value;
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MAXP; ++k) {
                 const int i = tid + k * NT;
                 if (i < A1) {
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) sF[f][i] = pre[k][f];
+                    for (int f = 0; f < 6; ++f) sF[f][i] = pre[S][k][f];
                 }
             }
-            if (zi < ze) fetch(zi + 1);
+            fetch(S, min(zi + 2, ze));  // past ze: a redundant reload of plane ze
             __syncthreads();
             const int o = (ty + 1) * W1 + (tx + 1);
 #pragma unroll
@@ -109,13 +134,22 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             }
             const int zo = zi - 1;
             if (zo >= zb && own) {
-                const float2 s = make_float2(am[0] * co.inv_dt0, am[1] * co.inv_dt0);
-                st2(glb(D.c), gidx(N, gx, gy, zo), s);
-                sr += (double)s.x;
-                si += (double)s.y;
+                const float2 sv = make_float2(am[0] * co.inv_dt0, am[1] * co.inv_dt0);
+                st2(glb(D.c), gidx(N, gx, gy, zo), sv);
+                sr += (double)sv.x;
+                si += (double)sv.y;
             }
 #pragma unroll
             for (int F = 0; F < 2; ++F) { am[F] = ac[F]; ac[F] = ap[F]; ap[F] = 0.f; }
+        };
+        fetch(0, zb - 1);
+        fetch(1, zb);
+        wait_vmem();
+        for (int zi = zb - 1;;) {
+            body(Ph<0>{}, zi);
+            if (++zi > ze) break;
+            body(Ph<1>{}, zi);
+            if (++zi > ze) break;
         }
         // FP64 partial sum of s for the k = 0 coefficient (fixed order)
 #pragma unroll
@@ -170,38 +204,43 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         float2 xm = {0.f, 0.f}, xc = {0.f, 0.f}, xp = {0.f, 0.f};
         float2 ym = {0.f, 0.f}, yc = {0.f, 0.f}, yp = {0.f, 0.f};
         float2 zmv = {0.f, 0.f}, zcv = {0.f, 0.f}, zpv = {0.f, 0.f};
-        // the next plane is loaded into registers while the current one is used
+        // planes of p are loaded two ahead into two register slots, with the
+        // v*, P* of the own cell at the plane finished by the slot's
+        // iteration; loads are unconditional (clamped positions and planes)
+        // so no register needs a merge copy
         constexpr int MAXP = (A1 + NT - 1) / NT;
-        float2 pre[MAXP];
-        // v*, P* of the own cell at the plane that is finished next iteration
-        float sv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, nv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        auto fetch = [&](int zi) {
-            const int gz = wrapi(zi, N);
+        unsigned off[MAXP];
 #pragma unroll
-            for (int k = 0; k < MAXP; ++k) {
-                const int i = tid + k * NT;
-                if (i < A1) {
-                    const int ex = wrapi(x0 - 1 + i % W1, N), ey = wrapi(y0 - 1 + i / W1, N);
-                    pre[k] = ld2(c, gidx(N, ex, ey, gz));
-                }
-            }
-            // unconditional (clamped, wrapped) so the registers need no merge
-            // copy, which would wait for the plane's loads issued just before
-            const size_t g = gidx(N, min(gx, N - 1), min(gy, N - 1), wrapi(zi - 1, N));
+        for (int k = 0; k < MAXP; ++k) {
+            const int i = min(tid + k * NT, A1 - 1);
+            off[k] = (unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N);
+        }
+        const unsigned own_off = 4u * ((unsigned)min(gy, N - 1) * (unsigned)N + (unsigned)min(gx, N - 1));
+        const size_t NN = (size_t)N * N;
+        float2 pre[2][MAXP];
+        float sv[2][6];
+        auto fetch = [&](int slot, int zi) {
+            const gptr<const float2> cz = c + (size_t)wrapi(zi, N) * NN;
 #pragma unroll
-            for (int f = 0; f < 6; ++f) nv[f] = src[f][g];
+            for (int k = 0; k < MAXP; ++k) pre[slot][k] = ld2(cz, off[k]);
+            // own-cell predictors of plane zi-1, the output of iteration zi
+            const size_t zo = (size_t)wrapi(max(zi - 1, zb), N) * NN;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) sv[slot][f] = ldb(src[f] + zo, own_off);
         };
-        fetch(zb - 1);
-        for (int zi = zb - 1; zi <= ze; ++zi) {
+        auto body = [&](auto ph, int zi) {
+            constexpr int S = decltype(ph)::value;
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MAXP; ++k) {
                 const int i = tid + k * NT;
-                if (i < A1) sp[i] = pre[k];
+                if (i < A1) sp[i] = pre[S][k];
             }
+            // this iteration's own-cell predictors, before the slot is reused
+            float svc[6];
 #pragma unroll
-            for (int f = 0; f < 6; ++f) sv[f] = nv[f];
-            if (zi < ze) fetch(zi + 1);
+            for (int f = 0; f < 6; ++f) svc[f] = sv[S][f];
+            fetch(S, min(zi + 2, ze));  // past ze: a redundant reload of plane ze
             __syncthreads();
             const int o = (ty + 1) * W1 + (tx + 1);
             float2 ax, ay, bz;
@@ -235,16 +274,25 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
             const int zo = zi - 1;
             if (zo >= zb && own) {
                 const size_t g = gidx(N, gx, gy, zo);
-                dst[0][g] = sv[0] - co.Dt * xm.x;
-                dst[1][g] = sv[1] - co.Dt * ym.x;
-                dst[2][g] = sv[2] - co.Dt * zmv.x;
-                dst[3][g] = sv[3] - co.Dt * xm.y;
-                dst[4][g] = sv[4] - co.Dt * ym.y;
-                dst[5][g] = sv[5] - co.Dt * zmv.y;
+                dst[0][g] = svc[0] - co.Dt * xm.x;
+                dst[1][g] = svc[1] - co.Dt * ym.x;
+                dst[2][g] = svc[2] - co.Dt * zmv.x;
+                dst[3][g] = svc[3] - co.Dt * xm.y;
+                dst[4][g] = svc[4] - co.Dt * ym.y;
+                dst[5][g] = svc[5] - co.Dt * zmv.y;
             }
             xm = xc; xc = xp; xp = make_float2(0.f, 0.f);
             ym = yc; yc = yp; yp = make_float2(0.f, 0.f);
             zmv = zcv; zcv = zpv; zpv = make_float2(0.f, 0.f);
+        };
+        fetch(0, zb - 1);
+        fetch(1, zb);
+        wait_vmem();
+        for (int zi = zb - 1;;) {
+            body(Ph<0>{}, zi);
+            if (++zi > ze) break;
+            body(Ph<1>{}, zi);
+            if (++zi > ze) break;
         }
     }
 }

@@ -103,19 +103,6 @@ __device__ __forceinline__ Der4 der_rows(const Rows& r, int fo, const Coef& co)
     return d;
 }
 
-// Byte-offset access to a plane: a uniform base plus a 32-bit per-lane byte
-// offset, the form of a global load or store with a scalar base address.
-__device__ __forceinline__ float ldb(gptr<const float> base, unsigned boff)
-{
-    return *(gptr<const float>)((gptr<const char>)base + boff);
-}
-__device__ __forceinline__ void stb(gptr<float> base, unsigned boff, float v)
-{
-    *(gptr<float>)((gptr<char>)base + boff) = v;
-}
-
-template <int K>
-using Ph = std::integral_constant<int, K>;
 
 template <int TX, int TY, int MINB>
 __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ drops, const Work* __restrict__ work,

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ttsh {
 
 constexpr int kMaxStages = 8;
@@ -115,6 +117,21 @@ __device__ __forceinline__ void st2(gptr<float2> p, size_t i, float2 v)
     t.y = v.y;
     *(gptr<fvec2>)(p + i) = t;
 }
+
+// Byte-offset access to a plane: a uniform base plus a 32-bit per-lane byte
+// offset, the form of a global load or store with a scalar base address.
+__device__ __forceinline__ float ldb(gptr<const float> base, unsigned boff)
+{
+    return *(gptr<const float>)((gptr<const char>)base + boff);
+}
+__device__ __forceinline__ void stb(gptr<float> base, unsigned boff, float v)
+{
+    *(gptr<float>)((gptr<char>)base + boff) = v;
+}
+
+// Compile-time phase of an unrolled z-march (register slot rotation).
+template <int K>
+using Ph = std::integral_constant<int, K>;
 
 // Waits for every outstanding vector-memory load (s_waitcnt vmcnt(0), the
 // export and LDS counters left at their maxima).  Issued at the end of a
