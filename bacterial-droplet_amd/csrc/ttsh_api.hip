@@ -19,10 +19,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ttsh3d.h"
@@ -128,6 +132,11 @@ struct Engine {
     double* d_redpart = nullptr;
     double* d_redout = nullptr;
     static constexpr int kRedBlocks = 256;
+    // diagnostics: centroid partial sums and result, one slice buffer
+    double* d_cpart = nullptr;
+    double* d_csum = nullptr;
+    float* d_slice = nullptr;
+    int slice_N = 0;
 
     // graphs, one per starting parity; the sizes present at capture
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -164,7 +173,11 @@ struct Engine {
         if (d_counts) (void)hipFree(d_counts);
         if (d_redpart) (void)hipFree(d_redpart);
         if (d_redout) (void)hipFree(d_redout);
+        if (d_cpart) (void)hipFree(d_cpart);
+        if (d_csum) (void)hipFree(d_csum);
+        if (d_slice) (void)hipFree(d_slice);
         d_drops = nullptr; d_counts = nullptr; d_redpart = nullptr; d_redout = nullptr;
+        d_cpart = nullptr; d_csum = nullptr; d_slice = nullptr; slice_N = 0;
         if (st) (void)hipStreamDestroy(st);
         st = nullptr;
     }
@@ -189,6 +202,71 @@ struct Engine {
         HIPCHK(hipMalloc(&d_counts, sizeof(int) * kNTables));
         HIPCHK(hipMalloc(&d_redpart, sizeof(double) * 3 * kRedBlocks));
         HIPCHK(hipMalloc(&d_redout, sizeof(double) * 3));
+        HIPCHK(hipMalloc(&d_cpart, sizeof(double) * 4 * kRedBlocks));
+        HIPCHK(hipMalloc(&d_csum, sizeof(double) * 4));
+        return TTSH_OK;
+    }
+
+    // Diagnostic launches on the engine stream for slot si: the centroid sums
+    // into sums (4 doubles), then (slice != null) plane z (< 0: the centroid
+    // plane) of phi, v, P and the vorticities into slice (9*N*N+1 floats).
+    int launch_diag(int si, double* sums, int z, float* slice)
+    {
+        Slot& sl = slots[si];
+        const float* phi = sl.drop.phi[sl.q];
+        HIPCHK(launch_centroid(phi, sl.N, d_cpart, kRedBlocks, sums, st));
+        if (slice) {
+            const float* v[3] = {sl.drop.v[0], sl.drop.v[1], sl.drop.v[2]};
+            const float* P[3] = {sl.drop.P[0], sl.drop.P[1], sl.drop.P[2]};
+            HIPCHK(launch_slice(phi, v, P, sl.N, z, sums, (float)sl.prm.h, slice, st));
+        }
+        return TTSH_OK;
+    }
+
+    int centroid(int si, ttsh_centroid* out)
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        HIPCHK(hipSetDevice(device));
+        int rc = launch_diag(si, d_csum, 0, nullptr);
+        if (rc) return rc;
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, d_csum, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        centroid_from_sums(h, out);
+        return TTSH_OK;
+    }
+
+    static void centroid_from_sums(const double h[4], ttsh_centroid* out)
+    {
+        // sums are (phi, phi*y, phi*x, phi*z); the snapshot axes are (y, x, z)
+        out->mass = h[0];
+        for (int a = 0; a < 3; ++a) {
+            out->c[a] = h[1 + a] / h[0];
+            out->idx[a] = std::isfinite(out->c[a]) ? (int32_t)out->c[a] : 0;  // truncation, as Python int()
+        }
+    }
+
+    int slice(int si, int z, float* out, int* z_used)
+    {
+        Slot& s = slots[si];
+        if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
+        if (z >= s.N) return fail(TTSH_EINVAL, "plane outside the box");
+        HIPCHK(hipSetDevice(device));
+        if (slice_N != s.N) {
+            if (d_slice) HIPCHK(hipFree(d_slice));
+            d_slice = nullptr;
+            HIPCHK(hipMalloc(&d_slice, sizeof(float) * (9 * (size_t)s.N * s.N + 1)));
+            slice_N = s.N;
+        }
+        int rc = launch_diag(si, d_csum, z, d_slice);
+        if (rc) return rc;
+        const size_t nn = (size_t)s.N * s.N;
+        HIPCHK(hipMemcpyAsync(out, d_slice, sizeof(float) * 9 * nn, hipMemcpyDeviceToHost, st));
+        float zf = (float)z;
+        if (z < 0) HIPCHK(hipMemcpyAsync(&zf, d_slice + 9 * nn, sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (z_used) *z_used = (int)zf;
         return TTSH_OK;
     }
 
@@ -678,6 +756,223 @@ struct Engine {
     }
 };
 
+// Exports of ttsh_run_ex.  At an export the state (and, for diagnostics, the
+// centroid sums and the centroid slice) is copied into one of two device
+// snapshot buffers on the compute stream; a second stream copies it to
+// pinned host memory once that copy is done; a host thread formats and
+// writes the files.  Stepping goes on meanwhile.  With both buffers in use
+// the next export waits for the older one.
+class SnapWriter {
+public:
+    ~SnapWriter() { (void)finish(); release(); }
+
+    int open(Engine* eng, int slot, const std::string& run_dir, int fl)
+    {
+        e = eng;
+        si = slot;
+        dir = run_dir;
+        flags = fl;
+        const Slot& sl = e->slots[si];
+        N = sl.N;
+        n = (size_t)N * N * N;
+        expo_data = sl.prm.ExpoData;
+        HIPCHK(hipSetDevice(e->device));
+        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        const size_t nn = (size_t)N * N;
+        for (int b = 0; b < 2; ++b) {
+            if (flags & (TTSH_RUN_TEXT | TTSH_RUN_BINARY)) {
+                HIPCHK(hipMalloc(&dsnap[b], sizeof(float) * 7 * n));
+                HIPCHK(hipHostMalloc((void**)&hsnap[b], sizeof(float) * 7 * n, hipHostMallocDefault));
+            }
+            if (flags & TTSH_RUN_DIAG) {
+                HIPCHK(hipMalloc(&dslice[b], sizeof(float) * (9 * nn + 1)));
+                HIPCHK(hipHostMalloc((void**)&hslice[b], sizeof(float) * (9 * nn + 1), hipHostMallocDefault));
+                HIPCHK(hipMalloc(&dsum[b], sizeof(double) * 4));
+                HIPCHK(hipHostMalloc((void**)&hsum[b], sizeof(double) * 4, hipHostMallocDefault));
+            }
+            HIPCHK(hipEventCreateWithFlags(&snap_ev[b], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&done_ev[b], hipEventDisableTiming));
+        }
+        if (flags & TTSH_RUN_DIAG) {
+            fdiag = std::fopen((dir + "/diag.dat").c_str(), "w");
+            far = std::fopen((dir + "/a_R.txt").c_str(), "w");
+            if (!fdiag || !far) return fail(TTSH_EIO, "cannot write diag.dat / a_R.txt in " + dir);
+        }
+        th = std::thread([this] { work(); });
+        return TTSH_OK;
+    }
+
+    // Export m of the state at time T after `steps` steps (main thread).
+    int submit(int m, double T, int64_t steps)
+    {
+        const int b = next;
+        next ^= 1;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return !busy[b] || err; });
+            if (err) return fail(err, errmsg);
+        }
+        const Slot& sl = e->slots[si];
+        hipStream_t st = e->st;
+        if (dsnap[b]) {
+            const float* src[7] = {sl.drop.v[0], sl.drop.v[1], sl.drop.v[2], sl.drop.P[0],
+                                   sl.drop.P[1], sl.drop.P[2], sl.drop.phi[sl.q]};
+            for (int q = 0; q < 7; ++q)
+                HIPCHK(hipMemcpyAsync(dsnap[b] + q * n, src[q], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        }
+        if (dslice[b]) {
+            int rc = e->launch_diag(si, dsum[b], -1, dslice[b]);
+            if (rc) return rc;
+        }
+        HIPCHK(hipEventRecord(snap_ev[b], st));
+        HIPCHK(hipStreamWaitEvent(cs, snap_ev[b], 0));
+        if (dsnap[b]) HIPCHK(hipMemcpyAsync(hsnap[b], dsnap[b], sizeof(float) * 7 * n, hipMemcpyDeviceToHost, cs));
+        if (dslice[b]) {
+            HIPCHK(hipMemcpyAsync(hslice[b], dslice[b], sizeof(float) * (9 * (size_t)N * N + 1), hipMemcpyDeviceToHost, cs));
+            HIPCHK(hipMemcpyAsync(hsum[b], dsum[b], sizeof(double) * 4, hipMemcpyDeviceToHost, cs));
+        }
+        HIPCHK(hipEventRecord(done_ev[b], cs));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            busy[b] = true;
+            q.push_back(Job{b, m, T, steps});
+        }
+        cv.notify_all();
+        if (flags & TTSH_RUN_SYNC) {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return !busy[b] || err; });
+            if (err) return fail(err, errmsg);
+        }
+        return TTSH_OK;
+    }
+
+    // Wait for every export to be written; the first error, if any.
+    int finish()
+    {
+        if (th.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                quit = true;
+            }
+            cv.notify_all();
+            th.join();
+        }
+        if (fdiag) { std::fclose(fdiag); fdiag = nullptr; }
+        if (far) { std::fclose(far); far = nullptr; }
+        if (err) return fail(err, errmsg);
+        return TTSH_OK;
+    }
+
+private:
+    struct Job {
+        int b, m;
+        double T;
+        int64_t steps;
+    };
+
+    void work()
+    {
+        (void)hipSetDevice(e->device);
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return quit || !q.empty(); });
+                if (q.empty()) break;
+                j = q.front();
+                q.pop_front();
+            }
+            int rc = write_job(j);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                busy[j.b] = false;
+                if (rc && !err) {
+                    err = rc;
+                    errmsg = g_last_error;
+                }
+            }
+            cv.notify_all();
+        }
+    }
+
+    int write_job(const Job& j)
+    {
+        const hipError_t he = hipEventSynchronize(done_ev[j.b]);
+        if (he != hipSuccess) return fail(TTSH_EHIP, std::string("snapshot copy failed: ") + hipGetErrorString(he));
+        const std::string stem = dir + "/conf_" + std::to_string(j.m);
+        int rc = TTSH_OK;
+        if (flags & TTSH_RUN_TEXT) {
+            const float* h = hsnap[j.b];
+            rc = write_conf_host((stem + ".dat").c_str(), N, expo_data, h + 6 * n, h + 3 * n, h);
+            if (rc) return rc;
+        }
+        if (flags & TTSH_RUN_BINARY) {
+            const float* h = hsnap[j.b];
+            const float* f[7] = {h, h + n, h + 2 * n, h + 3 * n, h + 4 * n, h + 5 * n, h + 6 * n};
+            rc = write_conf_bin_host((stem + ".bin").c_str(), N, j.m, j.T, j.steps, f);
+            if (rc) return rc;
+        }
+        if (flags & TTSH_RUN_DIAG) {
+            ttsh_centroid c;
+            Engine::centroid_from_sums(hsum[j.b], &c);
+            double R[120], ctr[2];
+            rc = contour_radius(N, hslice[j.b], 0.5, R, ctr);  // phi is field 0 of the slice
+            if (rc) return rc;
+            std::fprintf(fdiag, "%d %.15g %d %d %d %.17g %.17g %.17g %.17g\n", j.m, j.T, c.idx[0], c.idx[1], c.idx[2],
+                         c.c[0], c.c[1], c.c[2], c.mass);
+            for (int a = 0; a < 120; ++a) std::fprintf(far, a ? " %.18e" : "%.18e", R[a]);
+            std::fputc('\n', far);
+            if (std::fflush(fdiag) || std::fflush(far)) return fail(TTSH_EIO, "write failed: diag.dat / a_R.txt");
+        }
+        return TTSH_OK;
+    }
+
+    void release()
+    {
+        if (!e) return;
+        (void)hipSetDevice(e->device);
+        if (cs) (void)hipStreamSynchronize(cs);
+        for (int b = 0; b < 2; ++b) {
+            if (dsnap[b]) (void)hipFree(dsnap[b]);
+            if (hsnap[b]) (void)hipHostFree(hsnap[b]);
+            if (dslice[b]) (void)hipFree(dslice[b]);
+            if (hslice[b]) (void)hipHostFree(hslice[b]);
+            if (dsum[b]) (void)hipFree(dsum[b]);
+            if (hsum[b]) (void)hipHostFree(hsum[b]);
+            if (snap_ev[b]) (void)hipEventDestroy(snap_ev[b]);
+            if (done_ev[b]) (void)hipEventDestroy(done_ev[b]);
+            dsnap[b] = hsnap[b] = dslice[b] = hslice[b] = nullptr;
+            dsum[b] = hsum[b] = nullptr;
+            snap_ev[b] = done_ev[b] = nullptr;
+        }
+        if (cs) (void)hipStreamDestroy(cs);
+        cs = nullptr;
+    }
+
+    Engine* e = nullptr;
+    int si = 0, N = 0, flags = 0, expo_data = 1, next = 0;
+    size_t n = 0;
+    std::string dir;
+    hipStream_t cs = nullptr;
+    float* dsnap[2] = {nullptr, nullptr};
+    float* hsnap[2] = {nullptr, nullptr};
+    float* dslice[2] = {nullptr, nullptr};
+    float* hslice[2] = {nullptr, nullptr};
+    double* dsum[2] = {nullptr, nullptr};
+    double* hsum[2] = {nullptr, nullptr};
+    hipEvent_t snap_ev[2] = {nullptr, nullptr}, done_ev[2] = {nullptr, nullptr};
+    FILE* fdiag = nullptr;
+    FILE* far = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job> q;
+    bool busy[2] = {false, false};
+    bool quit = false;
+    int err = TTSH_OK;
+    std::string errmsg;
+};
+
 }  // namespace ttsh
 
 using namespace ttsh;
@@ -803,13 +1098,25 @@ int ttsh_write_conf(ttsh_sim* s, const char* path)
 
 int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done)
 {
+    return ttsh_run_ex(s, run_dir, TTSH_RUN_TEXT, steps_done);
+}
+
+int ttsh_run_ex(ttsh_sim* s, const char* run_dir, int flags, int64_t* steps_done)
+{
     if (!s || !run_dir) return fail(TTSH_EINVAL, "null argument");
+    if (flags & ~(TTSH_RUN_TEXT | TTSH_RUN_BINARY | TTSH_RUN_DIAG | TTSH_RUN_SYNC))
+        return fail(TTSH_EINVAL, "unknown run flag");
     Slot& sl = s->eng.slots[0];
     const ttsh_params& p = sl.prm;
-    const std::string dir(run_dir);
+    SnapWriter w;
+    const bool out = (flags & (TTSH_RUN_TEXT | TTSH_RUN_BINARY | TTSH_RUN_DIAG)) != 0;
     int rc;
-    if (p.T0 == 0) {
-        rc = ttsh_write_conf(s, (dir + "/conf_0.dat").c_str());
+    if (out) {
+        rc = w.open(&s->eng, 0, run_dir, flags);
+        if (rc) return rc;
+    }
+    if (p.T0 == 0 && out) {
+        rc = w.submit(0, sl.T, sl.steps);
         if (rc) return rc;
     }
     // evolve() (TTSH3D.cu:311-331): step, T += Dt, export on a dte crossing,
@@ -828,16 +1135,29 @@ int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done)
             if (rc) return rc;
             total += pending;
             pending = 0;
-            if (expo) {
-                const int te = (int)std::floor(T / p.dte);
-                rc = ttsh_write_conf(s, (dir + "/conf_" + std::to_string(te) + ".dat").c_str());
+            if (expo && out) {
+                rc = w.submit((int)std::floor(T / p.dte), sl.T, sl.steps);
                 if (rc) return rc;
             }
             if (stop) break;
         }
     }
     if (steps_done) *steps_done = total;
-    return ttsh_synchronize(s);
+    rc = ttsh_synchronize(s);
+    if (rc) return rc;
+    return w.finish();
+}
+
+int ttsh_get_centroid(ttsh_sim* s, ttsh_centroid* out)
+{
+    if (!s || !out) return fail(TTSH_EINVAL, "null argument");
+    return s->eng.centroid(0, out);
+}
+
+int ttsh_get_slice(ttsh_sim* s, int z, float* out, int* z_used)
+{
+    if (!s || !out) return fail(TTSH_EINVAL, "null argument");
+    return s->eng.slice(0, z, out, z_used);
 }
 
 // ---- batches -----------------------------------------------------------------

@@ -2,6 +2,7 @@
 // the conf_<m>.dat snapshot writer and the seeded initial conditions.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -12,6 +13,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ttsh3d.h"
@@ -348,38 +350,53 @@ namespace ttsh {
 // Format a double as std::fixed << setprecision(15) does (printf "%.15f").
 static inline int fmt15(char* out, double v) { return std::snprintf(out, 64, "%.15f", v); }
 
+// Text of one ExpoConf row i (y): the N*N lines of x, then z, in order.
+static void format_row(std::string& out, int N, int expo_data, int i, const float* phi, const float* P,
+                       const float* v)
+{
+    const size_t n = (size_t)N * N * N;
+    char tmp[64];
+    out.clear();
+    for (int j = 0; j < N; ++j) {
+        for (int k = 0; k < N; ++k) {
+            const size_t idx = ((size_t)k * N + i) * N + j;
+            if (expo_data == 1) {
+                const float vals[7] = {v[idx], v[n + idx], v[2 * n + idx], P[idx], P[n + idx], P[2 * n + idx], phi[idx]};
+                for (int q = 0; q < 7; ++q) {
+                    const int m = fmt15(tmp, (double)vals[q]);
+                    out.append(tmp, m);
+                    out.push_back(q < 6 ? ' ' : '\n');
+                }
+            } else if (expo_data == -1) {
+                const float vals[2] = {v[idx], v[n + idx]};
+                for (int q = 0; q < 2; ++q) {
+                    const int m = fmt15(tmp, (double)vals[q]);
+                    out.append(tmp, m);
+                    out.push_back(' ');
+                }
+            }
+        }
+    }
+}
+
 int write_conf_host(const char* path, int N, int expo_data, const float* phi, const float* P, const float* v)
 {
     FILE* f = std::fopen(path, "w");
     if (!f) return fail(TTSH_EIO, std::string("cannot write ") + path);
-    const size_t n = (size_t)N * N * N;
-    std::vector<char> buf;
-    buf.reserve((size_t)N * 8 * 24 + 64);
-    char tmp[64];
-    // ExpoConf loop order (TTSH3D.cu:975-981): i < Ny (y), j < Nx (x), k < Nz (z).
-    for (int i = 0; i < N; ++i) {
-        for (int j = 0; j < N; ++j) {
-            buf.clear();
-            for (int k = 0; k < N; ++k) {
-                const size_t idx = ((size_t)k * N + i) * N + j;
-                if (expo_data == 1) {
-                    const float vals[7] = {v[idx], v[n + idx], v[2 * n + idx], P[idx], P[n + idx], P[2 * n + idx],
-                                           phi[idx]};
-                    for (int q = 0; q < 7; ++q) {
-                        const int m = fmt15(tmp, (double)vals[q]);
-                        buf.insert(buf.end(), tmp, tmp + m);
-                        buf.push_back(q < 6 ? ' ' : '\n');
-                    }
-                } else if (expo_data == -1) {
-                    const float vals[2] = {v[idx], v[n + idx]};
-                    for (int q = 0; q < 2; ++q) {
-                        const int m = fmt15(tmp, (double)vals[q]);
-                        buf.insert(buf.end(), tmp, tmp + m);
-                        buf.push_back(' ');
-                    }
-                }
-            }
-            if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+    // ExpoConf loop order (TTSH3D.cu:975-981): i < Ny (y), j < Nx (x), k < Nz
+    // (z).  Rows i are formatted by a few threads at a time and written in
+    // order.
+    const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::string> rows(nt);
+    for (int i0 = 0; i0 < N; i0 += nt) {
+        const int cnt = std::min(nt, N - i0);
+        std::vector<std::thread> th;
+        for (int t = 1; t < cnt; ++t)
+            th.emplace_back([&, t] { format_row(rows[t], N, expo_data, i0 + t, phi, P, v); });
+        format_row(rows[0], N, expo_data, i0, phi, P, v);
+        for (auto& x : th) x.join();
+        for (int t = 0; t < cnt; ++t) {
+            if (!rows[t].empty() && std::fwrite(rows[t].data(), 1, rows[t].size(), f) != rows[t].size()) {
                 std::fclose(f);
                 return fail(TTSH_EIO, std::string("write failed: ") + path);
             }
@@ -389,7 +406,91 @@ int write_conf_host(const char* path, int N, int expo_data, const float* phi, co
     return TTSH_OK;
 }
 
+int write_conf_bin_host(const char* path, int N, int m, double T, int64_t steps, const float* const fields[7])
+{
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(TTSH_EIO, std::string("cannot write ") + path);
+    const size_t n = (size_t)N * N * N;
+    const int32_t hdr_i[2] = {N, m};
+    bool ok = std::fwrite("TTSH3DB1", 1, 8, f) == 8 && std::fwrite(hdr_i, sizeof(int32_t), 2, f) == 2 &&
+              std::fwrite(&T, sizeof(double), 1, f) == 1 && std::fwrite(&steps, sizeof(int64_t), 1, f) == 1;
+    for (int q = 0; q < 7 && ok; ++q) ok = std::fwrite(fields[q], sizeof(float), n, f) == n;
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) return fail(TTSH_EIO, std::string("write failed: ") + path);
+    return TTSH_OK;
+}
+
+// Level-contour radius R(theta) (characteristicVelocity.py:75-122 on the
+// centroid slice).  The reference takes the vertices of matplotlib's contour
+// paths; here the vertex set is every level crossing of a horizontal or
+// vertical grid edge, interpolated linearly, in row-major edge order.
+int contour_radius(int N, const float* field, double level, double* R, double* center)
+{
+    std::vector<double> px, py;
+    auto F = [&](int r, int c) { return (double)field[(size_t)r * N + c] - level; };
+    auto cross = [](double a, double b) { return (a < 0.0) != (b < 0.0); };
+    for (int r = 0; r < N; ++r) {
+        for (int c = 0; c < N; ++c) {
+            const double a = F(r, c);
+            if (c + 1 < N) {
+                const double b = F(r, c + 1);
+                if (cross(a, b)) {
+                    px.push_back(c + a / (a - b));
+                    py.push_back(r);
+                }
+            }
+            if (r + 1 < N) {
+                const double b = F(r + 1, c);
+                if (cross(a, b)) {
+                    px.push_back(c);
+                    py.push_back(r + a / (a - b));
+                }
+            }
+        }
+    }
+    if (px.empty()) return fail(TTSH_EINVAL, "the level is not crossed");
+    double cx = 0.0, cy = 0.0;
+    for (size_t q = 0; q < px.size(); ++q) {
+        cx += px[q];
+        cy += py[q];
+    }
+    cx /= (double)px.size();
+    cy /= (double)px.size();
+    std::vector<double> pr(px.size()), pt(px.size());
+    for (size_t q = 0; q < px.size(); ++q) {
+        const double dx = px[q] - cx, dy = py[q] - cy;
+        pr[q] = std::sqrt(dx * dx + dy * dy);
+        pt[q] = std::atan2(dy, dx);
+    }
+    const double pi = 3.14159265358979323846;
+    int a = 0;
+    for (int deg = -180; deg < 180; deg += 3, ++a) {
+        const double th = deg / 180.0 * pi;
+        size_t best = 0;
+        double bd = std::fabs(pt[0] - th);
+        for (size_t q = 1; q < pt.size(); ++q) {
+            const double d = std::fabs(pt[q] - th);
+            if (d < bd) {
+                bd = d;
+                best = q;
+            }
+        }
+        R[a] = pr[best];
+    }
+    if (center) {
+        center[0] = cx;
+        center[1] = cy;
+    }
+    return TTSH_OK;
+}
+
 }  // namespace ttsh
+
+extern "C" int ttsh_contour_radius(int N, const float* field, double level, double* R, double* center)
+{
+    if (!field || !R || N < 2) return ttsh::fail(TTSH_EINVAL, "bad argument");
+    return ttsh::contour_radius(N, field, level, R, center);
+}
 
 extern "C" int ttsh_write_conf_arrays(const char* path, int N, int ExpoData, const float* phi, const float* P,
                                       const float* v)

@@ -1,5 +1,6 @@
 // ttsh_internal.h -- library-internal helpers shared by the host translation units.
 #pragma once
+#include <cstdint>
 #include <string>
 
 namespace ttsh {
@@ -9,6 +10,11 @@ int fail(int code, const std::string& msg);
 int fft_plan(int N, int* rad, int maxst);
 bool supported_N(int N, std::string* why);
 int write_conf_host(const char* path, int N, int expo_data, const float* phi, const float* P, const float* v);
+// Binary sidecar: "TTSH3DB1", int32 N, int32 m, double T, int64 steps, then
+// fields[0..6] = vx vy vz px py pz phi as FP32 N^3 each.
+int write_conf_bin_host(const char* path, int N, int m, double T, int64_t steps, const float* const fields[7]);
+// Level-contour radius R(theta) at 120 angles of an N x N [row][col] field.
+int contour_radius(int N, const float* field, double level, double* R, double* center);
 // Tuned K_rhs variant, K_rhs z-depth and projection z-depth for size N.
 bool tune_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz);
 

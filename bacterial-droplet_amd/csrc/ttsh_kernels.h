@@ -48,6 +48,13 @@ bool fft_xy_fused(int N);
 // FFT_MODE_FWD transforms x then y, FFT_MODE_INV y then x.
 hipError_t launch_fft_xy(int N, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
                          hipStream_t st);
+// Diagnostics (k_diag.hip): FP64 sums of phi, phi*y, phi*x, phi*z -> out[4]
+// (part holds 4*nblk doubles); plane z of phi, v, P and the 4th-order
+// vorticities of v and P -> out[9][N][N] (out has 9*N*N+1 floats; z < 0 takes
+// the centroid plane from sums and stores it in out[9*N*N]).
+hipError_t launch_centroid(const float* phi, int N, double* part, int nblk, double* out, hipStream_t st);
+hipError_t launch_slice(const float* phi, const float* const v[3], const float* const P[3], int N, int z,
+                        const double* sums, float h, float* out, hipStream_t st);
 // FP64 reductions: per droplet sum(phi), sum|P|, sum|v|^2 -> out[3*d..].
 hipError_t launch_reduce(const Drop* drops, int ndrops, int parity, double* part, int nblk, double* out,
                          hipStream_t st);

@@ -15,6 +15,11 @@
 //     TTSH_SEED; default time(0) as the reference's seed, TTSH3D.cu:40);
 //   - InitCond == 0 (a legacy 2D file read of vx, vy, TTSH3D.cu:768-797) is
 //     rejected.
+// Output options (none changes the default, the reference's text snapshots):
+//   --binary    also write conf_<m>.bin (raw FP32 fields)
+//   --diag      also write diag.dat (centroid) and a_R.txt (contour radii)
+//   --no-text   skip the text snapshots
+// Snapshots are written by a host thread while the time loop continues.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -27,14 +32,16 @@
 
 static int usage()
 {
-    std::fprintf(stderr, "usage: ttsh3d [gpu_node] [--device D] [--seed S] [--exact-halo 0|1]\n");
+    std::fprintf(stderr,
+                 "usage: ttsh3d [gpu_node] [--device D] [--seed S] [--exact-halo 0|1] [--binary] [--diag] "
+                 "[--no-text]\n");
     return 2;
 }
 
 int main(int argc, char** argv)
 {
     const auto t0 = std::chrono::steady_clock::now();
-    int gpu_node = 1, device = 0, faithful = 1;
+    int gpu_node = 1, device = 0, faithful = 1, flags = TTSH_RUN_TEXT;
     const char* env_node = std::getenv("TTSH_GPU_NODE");
     if (env_node) gpu_node = std::atoi(env_node);
     unsigned long long seed = (unsigned long long)std::time(nullptr);
@@ -45,6 +52,9 @@ int main(int argc, char** argv)
         if (a == "--device" && i + 1 < argc) device = std::atoi(argv[++i]);
         else if (a == "--seed" && i + 1 < argc) seed = std::strtoull(argv[++i], nullptr, 10);
         else if (a == "--exact-halo" && i + 1 < argc) faithful = std::atoi(argv[++i]);
+        else if (a == "--binary") flags |= TTSH_RUN_BINARY;
+        else if (a == "--diag") flags |= TTSH_RUN_DIAG;
+        else if (a == "--no-text") flags &= ~TTSH_RUN_TEXT;
         else if (a == "-h" || a == "--help") return usage();
         else if (!a.empty() && a[0] != '-') gpu_node = std::atoi(a.c_str());
         else return usage();
@@ -77,7 +87,7 @@ int main(int argc, char** argv)
     rc = ttsh_create(&p, device, &sim);
     if (rc == TTSH_OK) rc = ttsh_set_state(sim, TTSH_F64, phi.data(), P.data(), v.data(), vs.data(), ps.data());
     int64_t steps = 0;
-    if (rc == TTSH_OK) rc = ttsh_run(sim, dir.c_str(), &steps);
+    if (rc == TTSH_OK) rc = ttsh_run_ex(sim, dir.c_str(), flags, &steps);
     if (rc != TTSH_OK) {
         std::fprintf(stderr, "ttsh3d: %s\n", ttsh_last_error());
         ttsh_destroy(sim);

@@ -39,6 +39,17 @@ class TTSHError(RuntimeError):
         self.code = code
 
 
+class Centroid(C.Structure):
+    """ttsh_centroid: idx and c are (y, x, z), the snapshot array axes."""
+    _fields_ = [("idx", C.c_int32 * 3), ("c", C.c_double * 3), ("mass", C.c_double)]
+
+
+# ttsh_run_ex output flags
+RUN_TEXT, RUN_BINARY, RUN_DIAG, RUN_SYNC = 1, 2, 4, 8
+# fields of ttsh_get_slice, in order
+SLICE_FIELDS = ("phi", "vx", "vy", "vz", "px", "py", "pz", "vort_v", "vort_p")
+
+
 class Params(C.Structure):
     """Mirror of struct ttsh_params (include/ttsh3d.h)."""
     _fields_ = ([(n, C.c_int32) for n in INPUT_FIELDS[:12]]
@@ -84,6 +95,10 @@ def lib():
         "ttsh_write_conf": (C.c_int, [vp, C.c_char_p]),
         "ttsh_write_conf_arrays": (C.c_int, [C.c_char_p, C.c_int, C.c_int, vp, vp, vp]),
         "ttsh_run": (C.c_int, [vp, C.c_char_p, C.POINTER(i64)]),
+        "ttsh_run_ex": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(i64)]),
+        "ttsh_get_centroid": (C.c_int, [vp, C.POINTER(Centroid)]),
+        "ttsh_get_slice": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_int)]),
+        "ttsh_contour_radius": (C.c_int, [C.c_int, vp, C.c_double, dp, dp]),
         "ttsh_get_clock": (C.c_int, [vp, C.POINTER(i64), dp]),
         "ttsh_profile": (C.c_int, [vp, i64, dp]),
         "ttsh_kernel_name": (C.c_char_p, [C.c_int]),
@@ -185,6 +200,31 @@ def init_droplet(N: int, R: float, xi: float, pmode: int = 0, seed: int = 0):
     return phi, P, v
 
 
+def contour_radius(field, level: float = 0.5):
+    """(R[120], (cx, cy)): ttsh_contour_radius of a [row][col] field."""
+    f = np.ascontiguousarray(field, np.float32)
+    assert f.ndim == 2 and f.shape[0] == f.shape[1]
+    R = np.empty(120, np.float64)
+    c = np.empty(2, np.float64)
+    _check(lib().ttsh_contour_radius(f.shape[0], _ptr(f), level, R.ctypes.data_as(C.POINTER(C.c_double)),
+                                     c.ctypes.data_as(C.POINTER(C.c_double))))
+    return R, (c[0], c[1])
+
+
+def read_conf_bin(path: str):
+    """Fields of a conf_<m>.bin sidecar: dict of vx..phi as (N, N, N) [z][y][x], and m, T, steps."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    assert raw[:8] == b"TTSH3DB1", "not a TTSH3D binary snapshot"
+    N, m = np.frombuffer(raw, np.int32, 2, 8)
+    T = np.frombuffer(raw, np.float64, 1, 16)[0]
+    steps = np.frombuffer(raw, np.int64, 1, 24)[0]
+    n = int(N) ** 3
+    data = np.frombuffer(raw, np.float32, 7 * n, 32).reshape(7, N, N, N)
+    names = ("vx", "vy", "vz", "px", "py", "pz", "phi")
+    return {k: data[i] for i, k in enumerate(names)}, int(m), float(T), int(steps)
+
+
 def droplet_bytes(N: int) -> int:
     return int(lib().ttsh_droplet_bytes(int(N)))
 
@@ -277,10 +317,28 @@ class Simulation:
     def write_conf(self, path: str):
         _check(lib().ttsh_write_conf(self._h, os.fsencode(path)))
 
-    def run(self, run_dir: str) -> int:
+    def run(self, run_dir: str, text: bool = True, binary: bool = False, diag: bool = False,
+            sync: bool = False) -> int:
+        """The reference's time loop with exports (ttsh_run_ex); returns the steps taken."""
+        flags = (RUN_TEXT if text else 0) | (RUN_BINARY if binary else 0) | (RUN_DIAG if diag else 0) \
+            | (RUN_SYNC if sync else 0)
         n = C.c_int64()
-        _check(lib().ttsh_run(self._h, os.fsencode(run_dir), C.byref(n)))
+        _check(lib().ttsh_run_ex(self._h, os.fsencode(run_dir), flags, C.byref(n)))
         return n.value
+
+    def centroid(self):
+        """(idx, c, mass): the phi-weighted centroid over the (y, x, z) axes."""
+        c = Centroid()
+        _check(lib().ttsh_get_centroid(self._h, C.byref(c)))
+        return tuple(c.idx), tuple(c.c), c.mass
+
+    def slice(self, z: int = -1):
+        """(fields, z): plane z (-1: the centroid plane) of SLICE_FIELDS, each [y][x]."""
+        N = self.N
+        out = np.empty((9, N, N), np.float32)
+        zu = C.c_int()
+        _check(lib().ttsh_get_slice(self._h, int(z), _ptr(out), C.byref(zu)))
+        return {k: out[i] for i, k in enumerate(SLICE_FIELDS)}, zu.value
 
 
 class Batch:

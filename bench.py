@@ -11,8 +11,9 @@ droplet (R = 80, xi = 3, noisy aster P, run_cuda.pl defaults).
 A "step" is one explicit-Euler step of the reference's EEuler
 (TTSH3D.cu:347-359) on the droplet, device-resident.  W warm-up steps, then
 exactly K timed steps between two synchronisations, launched as a caller
-launches them (HIP-graph replays, no instrumentation); value = N^3 * K / wall
-seconds / 1e6 (Mcell-steps/s).  Then K more steps with HIP events recorded on
+launches them (ttsh_step: the step's kernels queued back to back on one
+stream, no instrumentation); value = N^3 * K / wall seconds / 1e6
+(Mcell-steps/s).  Then K more steps with HIP events recorded on
 the step's stream between consecutive kernel kinds give each kernel's average
 duration (kernel_ms; the events add launch gaps, so this run's own wall time,
 ms_per_step_instrumented, is longer).
@@ -183,8 +184,8 @@ def main():
     sim.set_state(phi, P, v)
     sim.step(args.warmup)
     sim.synchronize()
-    # timed region: K steps as a caller runs them (HIP-graph replays of 16
-    # steps, no instrumentation)
+    # timed region: K steps as a caller runs them (ttsh_step, no
+    # instrumentation)
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.synchronize()

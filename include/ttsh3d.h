@@ -154,6 +154,56 @@ int ttsh_get_clock(ttsh_sim* s, int64_t* steps, double* T);
  * (2*N^3 interleaved re/im) that the last corrector used. */
 int ttsh_get_aux(ttsh_sim* s, int which, float* out);
 
+/* ---- diagnostics (replace the post-processing reads of full snapshots) ---- */
+
+/* New: the phase-field centroid as readDraw.py:30-52 and
+ * characteristicVelocity.py:16-39 compute it from a snapshot: the sums of
+ * phi*i, phi*j, phi*k over the snapshot array axes (i, j, k) = (y, x, z),
+ * divided by sum phi.  idx[] holds them truncated toward zero (Python int()),
+ * c[] before truncation, mass = sum phi.  FP64 sums on the device. */
+typedef struct {
+    int32_t idx[3]; /* (y, x, z) */
+    double c[3];    /* (y, x, z) */
+    double mass;
+} ttsh_centroid;
+int ttsh_get_centroid(ttsh_sim* s, ttsh_centroid* out);
+
+/* New: plane z (z < 0: the centroid plane idx[2], readDraw.py:220-235) of
+ * phi, vx, vy, vz, px, py, pz and the vorticities d_x vy - d_y vx and
+ * d_x py - d_y px (readDraw.py:236-237) with the 4th-order periodic
+ * differences of op.py:3-13.  out receives 9*N*N floats, field-major, each
+ * field [y][x]; *z_used (may be NULL) the plane. */
+int ttsh_get_slice(ttsh_sim* s, int z, float* out, int* z_used);
+
+/* New, host only: radius R(theta) of the level contour of an N x N field,
+ * as characteristicVelocity.py:75-122 extracts it from the centroid slice:
+ * the level crossings on every horizontal and vertical grid edge (linear
+ * interpolation, no periodic wrap), their mean as the centre, and for
+ * theta = -180, -177, ..., 177 degrees the distance of the crossing whose
+ * polar angle about the centre is nearest (the first in row-major edge
+ * order on ties).  field is [row][col] with x = col, y = row; R receives 120
+ * values, center (may be NULL) the centre (x, y).  TTSH_EINVAL if the level
+ * is never crossed. */
+int ttsh_contour_radius(int N, const float* field, double level, double* R, double* center);
+
+/* Output options of ttsh_run_ex (bitwise or). */
+#define TTSH_RUN_TEXT 1   /* conf_<m>.dat, the reference's text format */
+#define TTSH_RUN_BINARY 2 /* conf_<m>.bin: "TTSH3DB1", int32 N, int32 m, double T,
+                             int64 steps, then vx vy vz px py pz phi as FP32 N^3
+                             each, x fastest */
+#define TTSH_RUN_DIAG 4   /* per export: a line "m T iy ix iz cy cx cz mass" in
+                             diag.dat and the 120 radii of the 0.5 contour of the
+                             centroid slice of phi in a_R.txt (np.savetxt layout,
+                             characteristicVelocity.py:125-135) */
+#define TTSH_RUN_SYNC 8   /* finish each export before stepping on */
+
+/* New: ttsh_run with a choice of outputs.  Exports are written by a host
+ * thread from a device snapshot copied out on a second stream, so stepping
+ * continues while a snapshot is formatted and written (two snapshots in
+ * flight at most); ttsh_run_ex returns once every file is written.
+ * ttsh_run(s, d, n) is ttsh_run_ex(s, d, TTSH_RUN_TEXT, n). */
+int ttsh_run_ex(ttsh_sim* s, const char* run_dir, int flags, int64_t* steps_done);
+
 /* ---- initial conditions (host, FP64, seeded and reproducible) --------------- */
 
 /* InitCond == 1 of InitConf (TTSH3D.cu:798-825): uniform noise of amplitude
