@@ -2,6 +2,7 @@
 mix (bitwise: the kernels are deterministic and per-droplet), the step
 limits hold, and a diverging droplet is flagged without touching its
 neighbours."""
+import os
 import sys
 
 import numpy as np
@@ -86,3 +87,22 @@ def test_large_batch_fused_xy_equals_alone(ttsh):
                 d = np.abs(a.astype(np.float64) - c) / (np.abs(a).max() + 1e-30)
                 log(f"seed {seed}: max scaled diff {d.max():.3e}")
             assert np.array_equal(a, c), f"droplet seed {seed} differs between batch and alone"
+
+
+def test_sweep_equals_runs_alone(ttsh, tmp_path):
+    """A forEach.pl grid of 3 points run as one batch writes, per point, the
+    same conf_<m>.dat bytes as that point's own ttsh_run."""
+    from ttsh3d import sweep
+    log("[sweep_equals_runs_alone]")
+    pts = sweep.grid({"cA": [-1, -0.5], "La": [1, 1.5]}, {"N": 32, "Tend": 0.3, "dT": 0.01, "dTe": 0.1})[:3]
+    steps = sweep.run_sweep(pts, str(tmp_path / "sweep"), seed=40, log=log)
+    for i, (name, p) in enumerate(pts):
+        d = tmp_path / "alone" / name
+        d.mkdir(parents=True)
+        phi, P, v, vs, ps = ttsh.init_reference(p, 40 + i)
+        with ttsh.Simulation(p) as sim:
+            sim.set_state(phi, P, v, vs, ps)
+            assert sim.run(str(d)) == steps
+        for f in sorted(os.listdir(d)):
+            assert open(d / f, "rb").read() == open(tmp_path / "sweep" / name / f, "rb").read(), (name, f)
+        assert os.path.exists(tmp_path / "sweep" / name / "input.dat")
