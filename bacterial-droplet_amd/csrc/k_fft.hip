@@ -225,8 +225,6 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
     using X = Xch<N, AXIS>;
     __shared__ float2 sx[X::SIZE];
     __shared__ float2 stw[N];
-    __shared__ double red[2 * (P::NB / 64)];
-    __shared__ double k0[2];
 
     const int tid = threadIdx.x;
     const bool active = tid < NT;
@@ -261,28 +259,14 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
         }
         const gptr<float2> c = glb(D.c);
 
-        // FP64 k=0 coefficient for the Poisson pass (work item holding kx=ky=0)
+        // the Poisson pass's work item holding kx = ky = 0 sets the k = 0
+        // coefficient from the FP64 sum of s (k_proj_stats)
         const bool has0 = MODE == FFT_POISSON && wk.a == 0 && wk.b == 0;
-        if (MODE == FFT_POISSON && has0) {
-            double a = 0.0, b = 0.0;
-            for (int j = tid; j < D.npart; j += blockDim.x) {
-                a += glb(D.part)[2 * j];
-                b += glb(D.part)[2 * j + 1];
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                a += __shfl_down(a, off, 64);
-                b += __shfl_down(b, off, 64);
-            }
-            if ((tid & 63) == 0) { red[2 * (tid >> 6)] = a; red[2 * (tid >> 6) + 1] = b; }
-        }
-        __syncthreads();
-        if (MODE == FFT_POISSON && has0 && tid == 0) {
-            double a = 0.0, b = 0.0;
-            for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
-            k0[0] = a;
-            k0[1] = b;
-        }
+        // the first forward pass divides the imaginary part by sigma (a
+        // power of two: exact)
+        float isg = 1.f;
+        if constexpr (AXIS == 0 && MODE == FFT_FWD) isg = 1.f / glb(D.sigma)[0];
+        __syncthreads();  // stw
 
         constexpr int DIR0 = MODE == FFT_INV ? 1 : -1;
         float2 u[R1 > R2 ? R1 : R2];
@@ -290,6 +274,10 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
         if (active && t < R1) {
 #pragma unroll
             for (int n2 = 0; n2 < R2; ++n2) u[n2] = ld2(c, base + (size_t)(t + R1 * n2) * es);
+            if constexpr (AXIS == 0 && MODE == FFT_FWD) {
+#pragma unroll
+                for (int n2 = 0; n2 < R2; ++n2) u[n2].y *= isg;
+            }
             Dft<R2, DIR0>::run(u);
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) {
@@ -318,7 +306,7 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
                 for (int k1 = 0; k1 < R1; ++k1) {
                     const int kz = t + R2 * k1;
                     float2 s = u[k1];
-                    if (has0 && l == 0 && kz == 0) s = f2((float)k0[0], (float)k0[1]);
+                    if (has0 && l == 0 && kz == 0) s = f2((float)glb(D.stat)[0], (float)glb(D.stat)[1]);
                     const float m = poisson_factor(co, N, lx, ly, kz);
                     u[k1] = f2(s.x * m, s.y * m);
                 }
@@ -437,9 +425,15 @@ __global__ __launch_bounds__(XyCfg<N>::NT) void k_fft_xy(const Drop* __restrict_
         const gptr<fvec4> cp = (gptr<fvec4>)(glb(D.c) + (size_t)wk.b * N * N);
         __syncthreads();  // the previous item's stores have read the plane
         for (int i = tid; i < N; i += NT) stw[i] = ld2(glb(D.tw), i);
-        // load: two complex elements of one row per 16-byte access
+        // load: two complex elements of one row per 16-byte access; the
+        // forward pass divides the imaginary part by sigma (a power of two)
+        const float isg = MODE == FFT_FWD ? 1.f / glb(D.sigma)[0] : 1.f;
         for (int i = tid; i < N * N / 2; i += NT) {
-            const fvec4 q = cp[i];
+            fvec4 q = cp[i];
+            if constexpr (MODE == FFT_FWD) {
+                q.y *= isg;
+                q.w *= isg;
+            }
             const int y = (2 * i) / N, x = (2 * i) % N;
             pl[y * P + x] = f2(q.x, q.y);
             pl[y * P + x + 1] = f2(q.z, q.w);

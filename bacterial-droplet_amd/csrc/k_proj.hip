@@ -8,6 +8,17 @@
 // p_v + i p_p exactly, so K_div writes s = (D.v* + i D.P*)/dt0 and K_corr
 // reads p_v + i p_p after the inverse transforms (k_fft.hip).
 //
+// In FP32 the rounding of the larger half of s leaks into the smaller one.
+// An aster's div P* can exceed div v* by 10^2 and its P by 10^3 its v; left
+// unscaled, v then loses three digits in the first step.  k_proj_stats
+// picks a power of two sigma per droplet and step; the first forward pass
+// divides the imaginary part by it and K_corr multiplies p_p back (both
+// exact).  Cross-talk into v scales with |div P*|/(sigma |div v*|) relative
+// to |P|/|v|, into P with the inverse, so
+//   sigma^4 = (sum (div P*)^2 sum |P*|^2) / (sum (div v*)^2 sum |v*|^2)
+// gives both corrected fields the same relative share, rounded to 2^e,
+// |e| <= 30 (e = 0 when s has no imaginary part or a sum is undefined).
+//
 // Both are z-marching stencils: a work-group owns a TX x TY column and walks
 // a z-range, loading one plane (with a one-cell xy apron) per iteration into
 // LDS and adding the plane's contribution to the three output planes it
@@ -31,7 +42,7 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
 {
     constexpr int NT = TX * TY, W1 = TX + 2, A1 = W1 * (TY + 2);
     __shared__ float sF[6][A1];
-    __shared__ double red[2 * (NT / 64)];
+    __shared__ double red[kPartSums * (NT / 64)];
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int nwork = *nwork_p;
     for (int v = blockIdx.x; v < xcd_span(nwork); v += gridDim.x) {
@@ -58,7 +69,10 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         const bool own = gx < N && gy < N;
         const size_t NN = (size_t)N * N;
         float am[2] = {0.f, 0.f}, ac[2] = {0.f, 0.f}, ap[2] = {0.f, 0.f};
-        double sr = 0.0, si = 0.0;
+        // sums over this item's cells (see Drop::part): s in FP64 (the k = 0
+        // coefficient), the squares, which only set sigma, per thread in FP32
+        double q[kPartSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        float q2 = 0.f, q3 = 0.f, q4 = 0.f, q5 = 0.f;
         // Per field, the current and the previous-step buffer are addressed
         // from the lower of the two (uniform base lo[f]) with a per-lane byte
         // offset that adds the distance dh[f] to reach the higher one.  Every
@@ -78,11 +92,13 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         // previous step where the position lies outside the box in x or y
         constexpr int MAXP = (A1 + NT - 1) / NT;
         unsigned off[MAXP], oxy[MAXP][6];
+        float own_slot[MAXP];  // 1: the slot is a cell of this tile inside the box
 #pragma unroll
         for (int k = 0; k < MAXP; ++k) {
             const int i = min(tid + k * NT, A1 - 1);
             const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
             const bool sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
+            own_slot[k] = (tid + k * NT < A1 && ex >= x0 && ex < x0 + TX && ey >= y0 && ey < y0 + TY && !sxy) ? 1.f : 0.f;
             off[k] = 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
 #pragma unroll
             for (int f = 0; f < 6; ++f) oxy[k][f] = off[k] + ((sxy == prv_hi[f]) ? dh[f] : 0u);
@@ -102,6 +118,14 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         };
         auto body = [&](auto ph, int zi) {
             constexpr int S = decltype(ph)::value;
+            if (zi >= zb && zi < ze) {
+#pragma unroll
+                for (int k = 0; k < MAXP; ++k) {
+                    const float* u = pre[S][k];
+                    q4 += own_slot[k] * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                    q5 += own_slot[k] * (u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);
+                }
+            }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MAXP; ++k) {
@@ -136,8 +160,10 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             if (zo >= zb && own) {
                 const float2 sv = make_float2(am[0] * co.inv_dt0, am[1] * co.inv_dt0);
                 st2(glb(D.c), gidx(N, gx, gy, zo), sv);
-                sr += (double)sv.x;
-                si += (double)sv.y;
+                q[0] += (double)sv.x;
+                q[1] += (double)sv.y;
+                q2 += sv.x * sv.x;
+                q3 += sv.y * sv.y;
             }
 #pragma unroll
             for (int F = 0; F < 2; ++F) { am[F] = ac[F]; ac[F] = ap[F]; ap[F] = 0.f; }
@@ -151,20 +177,96 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             body(Ph<1>{}, zi);
             if (++zi > ze) break;
         }
-        // FP64 partial sum of s for the k = 0 coefficient (fixed order)
+        // the item's sums, reduced in a fixed order
+        q[2] = q2;
+        q[3] = q3;
+        q[4] = q4;
+        q[5] = q5;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-            sr += __shfl_down(sr, off, 64);
-            si += __shfl_down(si, off, 64);
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) q[e] += __shfl_down(q[e], off, 64);
         }
         __syncthreads();
-        if ((tid & 63) == 0) { red[2 * (tid >> 6)] = sr; red[2 * (tid >> 6) + 1] = si; }
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) red[kPartSums * (tid >> 6) + e] = q[e];
+        }
+        __syncthreads();
+        if (tid < kPartSums) {
+            double a = 0.0;
+            for (int i = 0; i < NT / 64; ++i) a += red[kPartSums * i + tid];
+            glb(D.part)[(size_t)kPartSums * wk.c + tid] = a;
+        }
+    }
+}
+
+// One work-group per droplet (work item (d, -, -)): the sums of the
+// divergence items in item order, then the k = 0 coefficient of s/sigma and
+// this step's sigma (derivation at the top of this file).
+constexpr int kStatsThreads = 256;
+
+// log2 x for x >= 0 (-inf at 0, +inf at inf): exponent plus the FP32 log2 of
+// the mantissa, enough for a rounded power of two
+__device__ __forceinline__ double log2_coarse(double x)
+{
+    if (!(x > 0.0)) return -INFINITY;
+    if (x == INFINITY) return INFINITY;
+    int ex;
+    const double m = frexp(x, &ex);
+    return (double)ex + (double)log2f((float)m);
+}
+
+__global__ __launch_bounds__(kStatsThreads) void k_proj_stats(const Drop* __restrict__ drops,
+                                                              const Work* __restrict__ work,
+                                                              const int* __restrict__ nwork_p, int kstep)
+{
+    constexpr int NW = kStatsThreads / 64;
+    __shared__ double red[kPartSums * NW];
+    __shared__ double tot[kPartSums];
+    const int tid = threadIdx.x;
+    const int nwork = *nwork_p;
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const Drop& D = drops[work[w].d];
+        if (kstep >= D.nsteps) continue;
+        const int np = D.npart;
+        const gptr<const double> pt = glb<const double>(D.part);
+        double q[kPartSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int j = tid; j < np; j += kStatsThreads) {
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) q[e] += pt[(size_t)kPartSums * j + e];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) q[e] += __shfl_down(q[e], off, 64);
+        }
+        __syncthreads();
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) red[kPartSums * (tid >> 6) + e] = q[e];
+        }
+        __syncthreads();
+        if (tid < kPartSums) {
+            double a = 0.0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) a += red[kPartSums * i + tid];
+            tot[tid] = a;
+        }
         __syncthreads();
         if (tid == 0) {
-            double a = 0.0, b = 0.0;
-            for (int i = 0; i < NT / 64; ++i) { a += red[2 * i]; b += red[2 * i + 1]; }
-            glb(D.part)[2 * wk.c] = a;
-            glb(D.part)[2 * wk.c + 1] = b;
+            double t[kPartSums];
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) t[e] = tot[e];
+            // log2 of 0 is -inf, of inf +inf; -inf + inf is NaN
+            const double l = 0.25 * ((log2_coarse(t[3]) + log2_coarse(t[5])) - (log2_coarse(t[2]) + log2_coarse(t[4])));
+            int e = 0;
+            if (t[3] > 0.0 && l == l) e = (int)rint(fmin(fmax(l, -30.0), 30.0));
+            const float sig = ldexpf(1.f, e);
+            glb(D.sigma)[0] = sig;
+            glb(D.stat)[0] = t[0];
+            glb(D.stat)[1] = ldexp(t[1], -e);
         }
     }
 }
@@ -189,6 +291,8 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         const int x0 = (wk.a % nbx) * TX, y0 = (wk.a / nbx) * TY;
         const int zb = wk.b & 0xffff, ze = wk.b >> 16;
         const gptr<const float2> c = glb<const float2>(D.c);
+        // p_p = sigma * Im(p): the power-of-two scale folded into Dt
+        const float dts = co.Dt * glb(D.sigma)[0];
         gptr<const float> src[6];
         gptr<float> dst[6];
 #pragma unroll
@@ -277,9 +381,9 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
                 dst[0][g] = svc[0] - co.Dt * xm.x;
                 dst[1][g] = svc[1] - co.Dt * ym.x;
                 dst[2][g] = svc[2] - co.Dt * zmv.x;
-                dst[3][g] = svc[3] - co.Dt * xm.y;
-                dst[4][g] = svc[4] - co.Dt * ym.y;
-                dst[5][g] = svc[5] - co.Dt * zmv.y;
+                dst[3][g] = svc[3] - dts * xm.y;
+                dst[4][g] = svc[4] - dts * ym.y;
+                dst[5][g] = svc[5] - dts * zmv.y;
             }
             xm = xc; xc = xp; xp = make_float2(0.f, 0.f);
             ym = yc; yc = yp; yp = make_float2(0.f, 0.f);
@@ -366,6 +470,14 @@ hipError_t launch_div(const Drop* drops, const Work* work, const int* nwork_p, i
     if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_div<kProjTX, kProjTY>), dim3(grid), dim3(kProjTX * kProjTY), 0, st, drops, work, nwork_p,
                        parity, kstep);
+    return hipGetLastError();
+}
+
+hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
+                             hipStream_t st)
+{
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_proj_stats, dim3(grid), dim3(kStatsThreads), 0, st, drops, work, nwork_p, kstep);
     return hipGetLastError();
 }
 

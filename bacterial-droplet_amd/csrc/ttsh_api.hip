@@ -5,8 +5,8 @@
 // step of the engine is eight kernel kinds, each launched once per step (the
 // FFT passes once per grid size present), each covering every active
 // droplet through a work table (see ttsh_dev.h):
-//   K_rhs, K_div, FFT x fwd, FFT y fwd, FFT z (fwd, -1/k^2, inv), FFT y inv,
-//   FFT x inv, K_corr
+//   K_rhs, K_div (with k_proj_stats, the per-droplet sums behind it), FFT x
+//   fwd, FFT y fwd, FFT z (fwd, -1/k^2, inv), FFT y inv, FFT x inv, K_corr
 // standing in for the reference's EEuler (TTSH3D.cu:347-359: 107 kernel
 // launches and 4 cuFFT transforms per step).
 //
@@ -50,9 +50,10 @@ enum { KK_RHS = 0, KK_DIV, KK_FX, KK_FY, KK_FZ, KK_FYI, KK_FXI, KK_CORR, KK_N };
 static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_xy", "K_fft_y", "K_fft_z_poisson",
                                              "K_fft_y_inv", "K_fft_xy_inv", "K_corr"};
 
-// Work tables: rhs, div/corr (shared), and per grid size the x/y-pass table,
-// the z-pass table and the z-plane table of the fused x+y pass.
-enum { T_RHS = 0, T_PROJ = 1, T_FFT0 = 2 };
+// Work tables: rhs, div/corr (shared), one item per droplet (the projection's
+// sums), and per grid size the x/y-pass table, the z-pass table and the
+// z-plane table of the fused x+y pass.
+enum { T_RHS = 0, T_PROJ = 1, T_STAT = 2, T_FFT0 = 3 };
 constexpr int kNTables = T_FFT0 + 3 * kNumSizes;
 static inline int t_xy(int s) { return T_FFT0 + 3 * s; }      // line groups of x or y passes
 static inline int t_z(int s) { return T_FFT0 + 3 * s + 1; }   // line groups of the z pass
@@ -77,7 +78,8 @@ size_t droplet_bytes(int N)
 {
     const size_t n = (size_t)N * N * N;
     return 20 * align256(n * sizeof(float)) + align256(n * sizeof(float2)) +
-           align256((size_t)proj_items_max(N) * 2 * sizeof(double));
+           align256((size_t)proj_items_max(N) * kPartSums * sizeof(double)) + align256(2 * sizeof(double)) +
+           align256(sizeof(float));
 }
 
 static int default_rhs_variant(int N) { return (N % 32 == 0 && N >= 64) ? RHS_32x8 : RHS_16x16; }
@@ -385,11 +387,15 @@ struct Engine {
         for (int q = 0; q < 2; ++q)
             for (int a = 0; a < 3; ++a) D.ps[q][a] = (float*)take(n * 4);
         D.c = (float2*)take(n * 8);
-        D.part = (double*)take((size_t)proj_items_max(N) * 16);
+        D.part = (double*)take((size_t)proj_items_max(N) * kPartSums * sizeof(double));
+        D.stat = (double*)take(2 * sizeof(double));
+        D.sigma = (float*)take(sizeof(float));
         D.faithful = p.faithful_halo ? 1 : 0;
         make_coef(p, D.co);
         HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
         HIPCHK(hipStreamSynchronize(st));
+        const float one = 1.f;  // sigma until the first step sets it
+        HIPCHK(hipMemcpy(D.sigma, &one, sizeof(float), hipMemcpyHostToDevice));
         s.q = engine_parity;
         s.steps = 0;
         s.limit = -1;
@@ -460,8 +466,9 @@ struct Engine {
                 const int zb = k * N / nzp, ze = (k + 1) * N / nzp;
                 for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), ord++});
             }
-            // the Poisson pass sums the divergence items' FP64 partial sums
+            // k_proj_stats sums the divergence items' FP64 partial sums
             h_drops[si].npart = ord;
+            w[T_STAT].push_back(Work{si, 0, 0, 0});
             const int sidx = size_index(N);
             const int L = fft_lines_per_group(N);
             const bool planes = fft_xy_fused(N);
@@ -523,6 +530,7 @@ struct Engine {
         HIPCHK(launch_rhs(rhs_variant, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
         HIPCHK(mark(KK_DIV));
         HIPCHK(launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
+        HIPCHK(launch_proj_stats(d_drops, tab[T_STAT].d, cnt + T_STAT, kstep, grid(T_STAT), st));
         // x, y, z-Poisson, y inverse, x inverse; sizes with the fused x+y
         // pass do x+y in the x slots and nothing in the y slots
         const int axes[5] = {0, 1, 2, 1, 0};
@@ -1080,6 +1088,10 @@ int ttsh_get_aux(ttsh_sim* s, int which, float* out)
     }
     if (which == 2) {
         HIPCHK(hipMemcpy(out, sl.drop.c, n * sizeof(float2), hipMemcpyDeviceToHost));
+        // p_p = sigma * Im, with the last step's power-of-two scale (exact)
+        float sig = 1.f;
+        HIPCHK(hipMemcpy(&sig, sl.drop.sigma, sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) out[2 * i + 1] *= sig;
         return TTSH_OK;
     }
     return fail(TTSH_EINVAL, "which must be 0, 1 or 2");

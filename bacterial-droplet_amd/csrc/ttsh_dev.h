@@ -13,7 +13,8 @@
 //                    parity holds the previous step's v*, P*, which the
 //                    divergence reads across the periodic boundary exactly as
 //                    the reference's stale halo does (TTSH3D.cu:366-371,405-407,665-667)
-//   c                complex FP32 N^3: (div v* + i div P*)/dt0, transformed in place
+//   c                complex FP32 N^3: (div v* + i div P*)/dt0, transformed in place;
+//                    the first forward pass scales the imaginary part by 1/sigma
 //
 // All kernels take a device table of droplet descriptors and a work table of
 // (droplet, tile...) entries, so one launch serves one droplet or a batch of
@@ -48,6 +49,8 @@ struct Coef {
     int   anchoring;  // A != 0: evaluate the anchoring term
 };
 
+constexpr int kPartSums = 6;
+
 struct Drop {
     int N;
     int nst;                 // FFT stages
@@ -59,8 +62,19 @@ struct Drop {
     float* vs[2][3];
     float* ps[2][3];
     float2* c;
-    double* part;            // FP64 partial sums of c (one per divergence work item)
+    // FP64 sums per divergence work item, kPartSums each: s.x, s.y, s.x^2,
+    // s.y^2 over the item's cells and |v*|^2, |P*|^2 over the same cells
+    double* part;
     int npart;
+    // Per step, from the sums of all items (k_proj_stats): stat[0..1] the
+    // k = 0 coefficient of s/sigma (FP64); sigma the power-of-two scale of
+    // the imaginary part.  The packed transform carries div v* and div P*/
+    // sigma, and K_corr multiplies p_p back by sigma: with sigma chosen so
+    // that the FP32 rounding of either half disturbs the other by the same
+    // relative amount in the corrected v and P, a droplet whose div P* is
+    // orders of magnitude above its div v* keeps the accuracy of v.
+    double* stat;
+    float* sigma;
     int faithful;            // 1: stale-halo divergence (the reference's semantics)
     int nsteps;              // steps this droplet takes in the current launch sequence
     Coef co;

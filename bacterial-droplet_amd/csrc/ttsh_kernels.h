@@ -31,6 +31,10 @@ void rhs_tile(int variant, int* tx, int* ty);
 // s = (D.v* + i D.P*)/dt0 with the stale-halo rule; FP64 partial sums of s.
 hipError_t launch_div(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,
                       hipStream_t st);
+// Per droplet (work item (d, -, -)): the partial sums of the divergence items
+// in item order -> stat (k = 0 coefficient) and sigma of this step.
+hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
+                             hipStream_t st);
 // v = v* - Dt D p_v, P = P* - Dt D p_p from p_v + i p_p.
 hipError_t launch_corr(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,
                        hipStream_t st);
