@@ -87,11 +87,21 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             lo[f] = prv_hi[f] ? cur[f] : prv[f];
             dh[f] = 4u * (unsigned)(prv_hi[f] ? prv[f] - cur[f] : cur[f] - prv[f]);
         }
+        // In the stale-halo mode, positions outside the box in x read the
+        // previous step's x = 0 / x = N-1 columns from the edge buffer (same
+        // z-stride as a field, so the same scalar base serves; K_corr writes
+        // each step's columns).
+        const bool xedge = D.faithful != 0;
+        const int eprev = D.eslot[pp];
+        unsigned eoff[6];  // byte distance from lo[f] to the edge buffer
+#pragma unroll
+        for (int f = 0; f < 6; ++f) eoff[f] = 4u * (unsigned)(glb<const float>(D.edge) - lo[f]);
         // in-plane byte offsets of this thread's apron positions (clamped, so
         // every thread loads every slot), within a plane taken from the
         // previous step where the position lies outside the box in x or y
         constexpr int MAXP = (A1 + NT - 1) / NT;
         unsigned off[MAXP], oxy[MAXP][6];
+        bool xwrap[MAXP];      // the slot reads the edge buffer
         float own_slot[MAXP];  // 1: the slot is a cell of this tile inside the box
 #pragma unroll
         for (int k = 0; k < MAXP; ++k) {
@@ -100,20 +110,33 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             const bool sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
             own_slot[k] = (tid + k * NT < A1 && ex >= x0 && ex < x0 + TX && ey >= y0 && ey < y0 + TY && !sxy) ? 1.f : 0.f;
             off[k] = 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
+            xwrap[k] = xedge && (ex < 0 || ex >= N);
+            // x = -1 wraps to the column x = N-1 (side 1), x = N to x = 0
+            const unsigned ecol = 4u * (unsigned)(((eprev * 6) * 2 + (ex < 0 ? 1 : 0)) * N + wrapi(ey, N));
 #pragma unroll
-            for (int f = 0; f < 6; ++f) oxy[k][f] = off[k] + ((sxy == prv_hi[f]) ? dh[f] : 0u);
+            for (int f = 0; f < 6; ++f)
+                oxy[k][f] = xwrap[k] ? eoff[f] + ecol + 4u * (unsigned)(f * 2 * N)
+                                     : off[k] + ((sxy == prv_hi[f]) ? dh[f] : 0u);
         }
         // planes are loaded two ahead, into two register slots; a plane
         // outside the box (wrapped in z) is read wholly from the previous step
         float pre[2][MAXP][6];
         auto fetch = [&](int slot, int zi) {
-            const bool wz = zi < 0 || zi >= N;
             const size_t zo = (size_t)wrapi(zi, N) * NN;
+            if (zi >= 0 && zi < N) {
 #pragma unroll
-            for (int f = 0; f < 6; ++f) {
-                const unsigned adj = prv_hi[f] ? dh[f] : 0u;  // whole plane stale
+                for (int f = 0; f < 6; ++f) {
 #pragma unroll
-                for (int k = 0; k < MAXP; ++k) pre[slot][k][f] = ldb(lo[f] + zo, wz ? off[k] + adj : oxy[k][f]);
+                    for (int k = 0; k < MAXP; ++k) pre[slot][k][f] = ldb(lo[f] + zo, oxy[k][f]);
+                }
+            } else {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    const unsigned adj = prv_hi[f] ? dh[f] : 0u;  // whole plane stale
+#pragma unroll
+                    for (int k = 0; k < MAXP; ++k)
+                        pre[slot][k][f] = ldb(lo[f] + zo, xwrap[k] ? oxy[k][f] : off[k] + adj);
+                }
             }
         };
         auto body = [&](auto ph, int zi) {
@@ -321,6 +344,11 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         }
         const unsigned own_off = 4u * ((unsigned)min(gy, N - 1) * (unsigned)N + (unsigned)min(gx, N - 1));
         const size_t NN = (size_t)N * N;
+        // In the stale-halo mode the cells at x = 0 and x = N-1 also store
+        // this step's v*, P* in the edge buffer: the next step's divergence
+        // reads them there (the layout of Drop::edge).
+        const bool ewr = D.faithful != 0 && own && (gx == 0 || gx == N - 1);
+        const unsigned ewo = 4u * (unsigned)(((D.eslot[parity] * 6) * 2 + (gx == 0 ? 0 : 1)) * N + min(gy, N - 1));
         float2 pre[2][MAXP];
         float sv[2][6];
         auto fetch = [&](int slot, int zi) {
@@ -384,6 +412,11 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
                 dst[3][g] = svc[3] - dts * xm.y;
                 dst[4][g] = svc[4] - dts * ym.y;
                 dst[5][g] = svc[5] - dts * zmv.y;
+                if (ewr) {
+                    const gptr<float> e = glb(D.edge) + (size_t)zo * NN;
+#pragma unroll
+                    for (int f = 0; f < 6; ++f) stb(e, ewo + 4u * (unsigned)(f * 2 * N), svc[f]);
+                }
             }
             xm = xc; xc = xp; xp = make_float2(0.f, 0.f);
             ym = yc; yc = yp; yp = make_float2(0.f, 0.f);
@@ -478,6 +511,33 @@ hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwo
 {
     if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_proj_stats, dim3(grid), dim3(kStatsThreads), 0, st, drops, work, nwork_p, kstep);
+    return hipGetLastError();
+}
+
+// One thread per (z, y): the two end columns of the six predictor fields.
+__global__ __launch_bounds__(256) void k_fill_edges(const float* __restrict__ v0, const float* __restrict__ v1,
+                                                    const float* __restrict__ v2, const float* __restrict__ p0,
+                                                    const float* __restrict__ p1, const float* __restrict__ p2,
+                                                    float* __restrict__ edge, int slot, int N)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * N) return;
+    const int y = i % N, z = i / N;
+    const size_t row = ((size_t)z * N + y) * N, NN = (size_t)N * N;
+    const float* src[6] = {v0, v1, v2, p0, p1, p2};
+    const gptr<float> e = glb(edge) + (size_t)z * NN;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        e[((slot * 6 + f) * 2 + 0) * N + y] = glb(src[f])[row];
+        e[((slot * 6 + f) * 2 + 1) * N + y] = glb(src[f])[row + N - 1];
+    }
+}
+
+hipError_t launch_fill_edges(const float* const vs[3], const float* const ps[3], float* edge, int slot, int N,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_edges, dim3((N * N + 255) / 256), dim3(256), 0, st, vs[0], vs[1], vs[2], ps[0], ps[1],
+                       ps[2], edge, slot, N);
     return hipGetLastError();
 }
 

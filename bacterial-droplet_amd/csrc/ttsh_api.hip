@@ -79,7 +79,7 @@ size_t droplet_bytes(int N)
     const size_t n = (size_t)N * N * N;
     return 20 * align256(n * sizeof(float)) + align256(n * sizeof(float2)) +
            align256((size_t)proj_items_max(N) * kPartSums * sizeof(double)) + align256(2 * sizeof(double)) +
-           align256(sizeof(float));
+           align256(sizeof(float)) + align256(n * sizeof(float));
 }
 
 static int default_rhs_variant(int N) { return (N % 32 == 0 && N >= 64) ? RHS_32x8 : RHS_16x16; }
@@ -390,6 +390,11 @@ struct Engine {
         D.part = (double*)take((size_t)proj_items_max(N) * kPartSums * sizeof(double));
         D.stat = (double*)take(2 * sizeof(double));
         D.sigma = (float*)take(sizeof(float));
+        // last: the divergence addresses it from the predictor buffers with
+        // a non-negative 32-bit byte offset
+        D.edge = (float*)take(n * 4);
+        D.eslot[0] = 0;
+        D.eslot[1] = 1;
         D.faithful = p.faithful_halo ? 1 : 0;
         make_coef(p, D.co);
         HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
@@ -426,6 +431,7 @@ struct Engine {
         for (auto& s : slots) {
             if (!s.used || s.q == engine_parity) continue;
             std::swap(s.drop.phi[0], s.drop.phi[1]);
+            std::swap(s.drop.eslot[0], s.drop.eslot[1]);
             for (int a = 0; a < 3; ++a) {
                 std::swap(s.drop.vs[0][a], s.drop.vs[1][a]);
                 std::swap(s.drop.ps[0][a], s.drop.ps[1][a]);
@@ -707,6 +713,12 @@ struct Engine {
         for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.vs[q ^ 1][a], vsp ? vsp : v, a * n);
         for (int a = 0; a < 3 && !rc; ++a) rc = put(s.drop.ps[q ^ 1][a], psp ? psp : P, a * n);
         if (rc) return rc;
+        {
+            const float* vs[3] = {s.drop.vs[q ^ 1][0], s.drop.vs[q ^ 1][1], s.drop.vs[q ^ 1][2]};
+            const float* ps[3] = {s.drop.ps[q ^ 1][0], s.drop.ps[q ^ 1][1], s.drop.ps[q ^ 1][2]};
+            HIPCHK(launch_fill_edges(vs, ps, s.drop.edge, s.drop.eslot[q ^ 1], s.N, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
         s.steps = 0;
         s.T = s.prm.T0;
         tables_dirty = true;

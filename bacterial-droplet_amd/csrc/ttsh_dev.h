@@ -75,6 +75,14 @@ struct Drop {
     // orders of magnitude above its div v* keeps the accuracy of v.
     double* stat;
     float* sigma;
+    // x = 0 and x = N-1 columns of v*, P* per parity slot, laid out with a
+    // field's z-stride: plane z holds at ((slot*6 + f)*2 + side)*N + y the
+    // value of field f (vx..vz, Px..Pz) at x = side ? N-1 : 0, row y.  The
+    // stale-halo divergence reads the previous step's columns here instead
+    // of fetching a whole cache line of the previous predictor per row end.
+    // eslot[p] is the slot of the predictors in vs[p], ps[p].
+    float* edge;
+    int eslot[2];
     int faithful;            // 1: stale-halo divergence (the reference's semantics)
     int nsteps;              // steps this droplet takes in the current launch sequence
     Coef co;

@@ -35,6 +35,10 @@ hipError_t launch_div(const Drop* drops, const Work* work, const int* nwork_p, i
 // in item order -> stat (k = 0 coefficient) and sigma of this step.
 hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
                              hipStream_t st);
+// The x = 0 and x = N-1 columns of vs[3], ps[3] into edge slot `slot`
+// (the layout of Drop::edge).
+hipError_t launch_fill_edges(const float* const vs[3], const float* const ps[3], float* edge, int slot, int N,
+                             hipStream_t st);
 // v = v* - Dt D p_v, P = P* - Dt D p_p from p_v + i p_p.
 hipError_t launch_corr(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,
                        hipStream_t st);
