@@ -139,8 +139,9 @@ int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done);
 /* New (the reference only prints host CPU time, TTSH3D.cu:266,284-286): runs n
  * steps like ttsh_step with a HIP event recorded between consecutive kernels
  * on the handle's stream; kernel_ms[k] receives the summed milliseconds of
- * kernel kind k (8 kinds, names from ttsh_kernel_name: K_rhs, K_div, the
- * five FFT passes, K_corr).  Synchronises once per 256 steps. */
+ * kernel kind k (8 kinds, names from ttsh_kernel_name: K_rhs, K_div (with
+ * the per-droplet reduction of its sums that follows it), the five FFT
+ * passes, K_corr).  Synchronises once per 256 steps. */
 int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[8]);
 
 /* Name of kernel kind k of ttsh_profile ("" outside 0..7). */
