@@ -96,19 +96,35 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         unsigned eoff[6];  // byte distance from lo[f] to the edge buffer
 #pragma unroll
         for (int f = 0; f < 6; ++f) eoff[f] = 4u * (unsigned)(glb<const float>(D.edge) - lo[f]);
-        // in-plane byte offsets of this thread's apron positions (clamped, so
-        // every thread loads every slot), within a plane taken from the
-        // previous step where the position lies outside the box in x or y
-        constexpr int MAXP = (A1 + NT - 1) / NT;
+        // in-plane byte offsets of this thread's positions, within a plane
+        // taken from the previous step where the position lies outside the
+        // box in x or y.  Slot 0 is the thread's own cell, so a wave loads
+        // two whole rows (aligned cache lines); slot 1 is the one-cell ring
+        // around the tile (top row, bottom row, left and right column),
+        // clamped so every thread loads it.
+        static_assert(A1 - NT == 2 * W1 + 2 * TY, "slot 1 holds the ring");
+        constexpr int MAXP = 2;
         unsigned off[MAXP], oxy[MAXP][6];
+        int lds_i[MAXP];       // the position's LDS index (-1: nothing to store)
         bool xwrap[MAXP];      // the slot reads the edge buffer
         float own_slot[MAXP];  // 1: the slot is a cell of this tile inside the box
 #pragma unroll
         for (int k = 0; k < MAXP; ++k) {
-            const int i = min(tid + k * NT, A1 - 1);
+            int i;
+            if (k == 0) {
+                i = (ty + 1) * W1 + (tx + 1);
+                lds_i[k] = i;
+            } else {
+                const int r = min(tid, 2 * W1 + 2 * TY - 1);
+                if (r < W1) i = r;                                      // row y0-1
+                else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);      // row y0+TY
+                else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;    // column x0-1
+                else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);         // column x0+TX
+                lds_i[k] = tid < 2 * W1 + 2 * TY ? i : -1;
+            }
             const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
             const bool sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
-            own_slot[k] = (tid + k * NT < A1 && ex >= x0 && ex < x0 + TX && ey >= y0 && ey < y0 + TY && !sxy) ? 1.f : 0.f;
+            own_slot[k] = (k == 0 && !sxy) ? 1.f : 0.f;
             off[k] = 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
             xwrap[k] = xedge && (ex < 0 || ex >= N);
             // x = -1 wraps to the column x = N-1 (side 1), x = N to x = 0
@@ -152,8 +168,8 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MAXP; ++k) {
-                const int i = tid + k * NT;
-                if (i < A1) {
+                const int i = lds_i[k];
+                if (i >= 0) {
 #pragma unroll
                     for (int f = 0; f < 6; ++f) sF[f][i] = pre[S][k][f];
                 }
@@ -335,11 +351,26 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
         // v*, P* of the own cell at the plane finished by the slot's
         // iteration; loads are unconditional (clamped positions and planes)
         // so no register needs a merge copy
-        constexpr int MAXP = (A1 + NT - 1) / NT;
+        // slot 0: the own cell (whole aligned rows per wave); slot 1: the
+        // one-cell ring around the tile, clamped so every thread loads it
+        static_assert(A1 - NT == 2 * W1 + 2 * TY, "slot 1 holds the ring");
+        constexpr int MAXP = 2;
         unsigned off[MAXP];
+        int lds_i[MAXP];  // the position's LDS index (-1: nothing to store)
 #pragma unroll
         for (int k = 0; k < MAXP; ++k) {
-            const int i = min(tid + k * NT, A1 - 1);
+            int i;
+            if (k == 0) {
+                i = (ty + 1) * W1 + (tx + 1);
+                lds_i[k] = i;
+            } else {
+                const int r = min(tid, 2 * W1 + 2 * TY - 1);
+                if (r < W1) i = r;                                      // row y0-1
+                else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);      // row y0+TY
+                else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;    // column x0-1
+                else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);         // column x0+TX
+                lds_i[k] = tid < 2 * W1 + 2 * TY ? i : -1;
+            }
             off[k] = (unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N);
         }
         const unsigned own_off = 4u * ((unsigned)min(gy, N - 1) * (unsigned)N + (unsigned)min(gx, N - 1));
@@ -365,8 +396,8 @@ __global__ __launch_bounds__(TX * TY) void k_corr(const Drop* __restrict__ drops
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MAXP; ++k) {
-                const int i = tid + k * NT;
-                if (i < A1) sp[i] = pre[S][k];
+                const int i = lds_i[k];
+                if (i >= 0) sp[i] = pre[S][k];
             }
             // this iteration's own-cell predictors, before the slot is reused
             float svc[6];
