@@ -115,6 +115,9 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
     constexpr int MP2 = (A2 + NT - 1) / NT;  // phi, P positions per thread
     constexpr int MP1 = (A1 + NT - 1) / NT;  // v positions per thread
     static_assert(NRING <= NT, "one ring position per thread at most");
+    // load slots: own cell + one apron ring, where the ring fits one slot
+    constexpr bool RING2 = MP2 == 2 && A2 - NT <= NT;
+    constexpr bool RING1 = MP1 == 2 && A1 - NT <= NT;
     // per position: phi, Px, Py, Pz x (f, dz, sz) | vx, vy, vz x (f, dz, sz) |
     // Sigma(9, [3a+b]), mu(3), mu0 of one plane
     constexpr int SP = 13, SV = 9, SS = 13;
@@ -152,16 +155,47 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
         const bool own = (gx < N) && (gy < N);
         const unsigned gown = 4u * ((unsigned)gy * (unsigned)N + (unsigned)gx);
 
-        // in-plane byte offsets of this thread's phi/P and v positions
+        // in-plane byte offsets of this thread's phi/P and v positions, and
+        // their LDS indices (-1: a clamped duplicate, not stored).  Where a
+        // ring fits one slot, slot 0 is the own cell (a wave loads whole
+        // aligned rows) and slot 1 the apron ring around the tile;
+        // otherwise positions follow the LDS order.
         unsigned off2[MP2], off1[MP1];
+        int lds2[MP2], lds1[MP1];
 #pragma unroll
         for (int k = 0; k < MP2; ++k) {
-            const int i = min(tid + k * NT, A2 - 1);
+            int i = min(tid + k * NT, A2 - 1), l = tid + k * NT < A2 ? i : -1;
+            if constexpr (RING2) {
+                if (k == 0) {
+                    i = l = o2c;
+                } else {
+                    const int r = min(tid, A2 - NT - 1);
+                    if (r < 2 * W2) i = r;                                                  // rows y0-2, y0-1
+                    else if (r < 4 * W2) i = (TY + 2) * W2 + (r - 2 * W2);                  // rows y0+TY, +TY+1
+                    else if (r < 4 * W2 + 2 * TY) i = (2 + (r - 4 * W2) / 2) * W2 + (r - 4 * W2) % 2;
+                    else i = (2 + (r - 4 * W2 - 2 * TY) / 2) * W2 + W2 - 2 + (r - 4 * W2 - 2 * TY) % 2;
+                    l = tid < A2 - NT ? i : -1;
+                }
+            }
+            lds2[k] = l;
             off2[k] = 4u * ((unsigned)wrapi(y0 - 2 + i / W2, N) * (unsigned)N + (unsigned)wrapi(x0 - 2 + i % W2, N));
         }
 #pragma unroll
         for (int k = 0; k < MP1; ++k) {
-            const int i = min(tid + k * NT, A1 - 1);
+            int i = min(tid + k * NT, A1 - 1), l = tid + k * NT < A1 ? i : -1;
+            if constexpr (RING1) {
+                if (k == 0) {
+                    i = l = o1c;
+                } else {
+                    const int r = min(tid, A1 - NT - 1);
+                    if (r < W1) i = r;                                       // row y0-1
+                    else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);       // row y0+TY
+                    else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;     // column x0-1
+                    else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);          // column x0+TX
+                    l = tid < A1 - NT ? i : -1;
+                }
+            }
+            lds1[k] = l;
             off1[k] = 4u * ((unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N));
         }
         // three z-planes per position in registers, in slots rotating with the
@@ -227,8 +261,9 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < MP2; ++k) {
-                const int i = ti + k * NT;
-                if (i < A2) {
+                int i = lds2[k];
+                asm volatile("" : "+v"(i));
+                if (i >= 0) {
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
                         const float a = in[k][f][S0], b = in[k][f][S1], c = in[k][f][S2];
@@ -241,8 +276,9 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             }
 #pragma unroll
             for (int k = 0; k < MP1; ++k) {
-                const int i = ti + k * NT;
-                if (i < A1) {
+                int i = lds1[k];
+                asm volatile("" : "+v"(i));
+                if (i >= 0) {
 #pragma unroll
                     for (int f = 0; f < 3; ++f) {
                         const float a = vin[k][f][S0], b = vin[k][f][S1], c = vin[k][f][S2];
