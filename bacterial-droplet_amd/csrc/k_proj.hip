@@ -93,86 +93,95 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
         // each step's columns).
         const bool xedge = D.faithful != 0;
         const int eprev = D.eslot[pp];
-        unsigned eoff[6];  // byte distance from lo[f] to the edge buffer
+        // Loads: the thread's own cell, one per field (a wave loads two whole
+        // aligned rows), and the one-cell ring around the tile (top row,
+        // bottom row, left and right column) as RING x 6 (position, field)
+        // pairs spread over two loads per thread.  Ring loads address every
+        // field from gb, the lowest of the droplet's predictor buffers (one
+        // allocation, with the edge buffer after them: 32-bit offsets).
+        constexpr int RING = 2 * W1 + 2 * TY;
+        static_assert(A1 - NT == RING && 6 * RING <= 2 * NT, "two ring loads per thread");
+        gptr<const float> gb = lo[0];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) eoff[f] = 4u * (unsigned)(glb<const float>(D.edge) - lo[f]);
-        // in-plane byte offsets of this thread's positions, within a plane
-        // taken from the previous step where the position lies outside the
-        // box in x or y.  Slot 0 is the thread's own cell, so a wave loads
-        // two whole rows (aligned cache lines); slot 1 is the one-cell ring
-        // around the tile (top row, bottom row, left and right column),
-        // clamped so every thread loads it.
-        static_assert(A1 - NT == 2 * W1 + 2 * TY, "slot 1 holds the ring");
-        constexpr int MAXP = 2;
-        unsigned off[MAXP], oxy[MAXP][6];
-        int lds_i[MAXP];       // the position's LDS index (-1: nothing to store)
-        bool xwrap[MAXP];      // the slot reads the edge buffer
-        float own_slot[MAXP];  // 1: the slot is a cell of this tile inside the box
-#pragma unroll
-        for (int k = 0; k < MAXP; ++k) {
-            int i;
-            if (k == 0) {
-                i = (ty + 1) * W1 + (tx + 1);
-                lds_i[k] = i;
-            } else {
-                const int r = min(tid, 2 * W1 + 2 * TY - 1);
-                if (r < W1) i = r;                                      // row y0-1
-                else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);      // row y0+TY
-                else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;    // column x0-1
-                else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);         // column x0+TX
-                lds_i[k] = tid < 2 * W1 + 2 * TY ? i : -1;
+        for (int f = 1; f < 6; ++f) gb = lo[f] < gb ? lo[f] : gb;
+        auto pos = [&](int i, bool& sxy, int& ex, int& ey) {
+            ex = x0 - 1 + i % W1;
+            ey = y0 - 1 + i / W1;
+            sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
+            return 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
+        };
+        // byte offset of field f's value at position (ex, ey) from base b:
+        // the current or previous buffer (stale: outside the box, or the
+        // whole plane when wz), or the edge buffer for x outside the box
+        auto foff = [&](int f, gptr<const float> b, unsigned o, bool stale, int ex, int ey) {
+            if (xedge && (ex < 0 || ex >= N)) {
+                const int col = ((eprev * 6 + f) * 2 + (ex < 0 ? 1 : 0)) * N + wrapi(ey, N);
+                return 4u * (unsigned)(glb<const float>(D.edge) - b) + 4u * (unsigned)col;
             }
-            const int ex = x0 - 1 + i % W1, ey = y0 - 1 + i / W1;
-            const bool sxy = ex < 0 || ex >= N || ey < 0 || ey >= N;
-            own_slot[k] = (k == 0 && !sxy) ? 1.f : 0.f;
-            off[k] = 4u * ((unsigned)wrapi(ey, N) * (unsigned)N + (unsigned)wrapi(ex, N));
-            xwrap[k] = xedge && (ex < 0 || ex >= N);
-            // x = -1 wraps to the column x = N-1 (side 1), x = N to x = 0
-            const unsigned ecol = 4u * (unsigned)(((eprev * 6) * 2 + (ex < 0 ? 1 : 0)) * N + wrapi(ey, N));
+            return 4u * (unsigned)(lo[f] - b) + o + ((stale == prv_hi[f]) ? dh[f] : 0u);
+        };
+        unsigned own_o[6], own_w[6];  // own cell: normal plane, whole-plane-stale plane
+        bool own_in;
+        {
+            bool sxy;
+            int ex, ey;
+            const unsigned o = pos((ty + 1) * W1 + (tx + 1), sxy, ex, ey);
+            own_in = !sxy;
 #pragma unroll
-            for (int f = 0; f < 6; ++f)
-                oxy[k][f] = xwrap[k] ? eoff[f] + ecol + 4u * (unsigned)(f * 2 * N)
-                                     : off[k] + ((sxy == prv_hi[f]) ? dh[f] : 0u);
+            for (int f = 0; f < 6; ++f) {
+                own_o[f] = foff(f, lo[f], o, sxy, ex, ey);
+                own_w[f] = foff(f, lo[f], o, true, ex, ey);
+            }
+        }
+        unsigned ring_o[2], ring_w[2];
+        int ring_l[2];  // flat LDS index f*A1 + i (-1: past the last pair)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int j = min(tid + m * NT, 6 * RING - 1), f = j / RING, r = j % RING;
+            int i;
+            if (r < W1) i = r;                                      // row y0-1
+            else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);      // row y0+TY
+            else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;    // column x0-1
+            else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);         // column x0+TX
+            bool sxy;
+            int ex, ey;
+            const unsigned o = pos(i, sxy, ex, ey);
+            ring_o[m] = foff(f, gb, o, sxy, ex, ey);
+            ring_w[m] = foff(f, gb, o, true, ex, ey);
+            ring_l[m] = tid + m * NT < 6 * RING ? f * A1 + i : -1;
         }
         // planes are loaded two ahead, into two register slots; a plane
         // outside the box (wrapped in z) is read wholly from the previous step
-        float pre[2][MAXP][6];
+        float pre[2][8];
         auto fetch = [&](int slot, int zi) {
             const size_t zo = (size_t)wrapi(zi, N) * NN;
             if (zi >= 0 && zi < N) {
 #pragma unroll
-                for (int f = 0; f < 6; ++f) {
+                for (int f = 0; f < 6; ++f) pre[slot][f] = ldb(lo[f] + zo, own_o[f]);
 #pragma unroll
-                    for (int k = 0; k < MAXP; ++k) pre[slot][k][f] = ldb(lo[f] + zo, oxy[k][f]);
-                }
+                for (int m = 0; m < 2; ++m) pre[slot][6 + m] = ldb(gb + zo, ring_o[m]);
             } else {
 #pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    const unsigned adj = prv_hi[f] ? dh[f] : 0u;  // whole plane stale
+                for (int f = 0; f < 6; ++f) pre[slot][f] = ldb(lo[f] + zo, own_w[f]);
 #pragma unroll
-                    for (int k = 0; k < MAXP; ++k)
-                        pre[slot][k][f] = ldb(lo[f] + zo, xwrap[k] ? oxy[k][f] : off[k] + adj);
-                }
+                for (int m = 0; m < 2; ++m) pre[slot][6 + m] = ldb(gb + zo, ring_w[m]);
             }
         };
         auto body = [&](auto ph, int zi) {
             constexpr int S = decltype(ph)::value;
-            if (zi >= zb && zi < ze) {
-#pragma unroll
-                for (int k = 0; k < MAXP; ++k) {
-                    const float* u = pre[S][k];
-                    q4 += own_slot[k] * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-                    q5 += own_slot[k] * (u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);
-                }
+            if (zi >= zb && zi < ze && own_in) {
+                const float* u = pre[S];
+                q4 += u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                q5 += u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
             }
             __syncthreads();
+            {
+                const int i = (ty + 1) * W1 + (tx + 1);
 #pragma unroll
-            for (int k = 0; k < MAXP; ++k) {
-                const int i = lds_i[k];
-                if (i >= 0) {
+                for (int f = 0; f < 6; ++f) sF[f][i] = pre[S][f];
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) sF[f][i] = pre[S][k][f];
-                }
+                for (int m = 0; m < 2; ++m)
+                    if (ring_l[m] >= 0) (&sF[0][0])[ring_l[m]] = pre[S][6 + m];
             }
             fetch(S, min(zi + 2, ze));  // past ze: a redundant reload of plane ze
             __syncthreads();
