@@ -112,12 +112,11 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
     constexpr int W2 = TX + 4, H2 = TY + 4, A2 = W2 * H2;  // phi, P, v planes: 2-cell apron layout
     constexpr int W1 = TX + 2, H1 = TY + 2, A1 = W1 * H1;  // v and intermediates: 1-cell apron
     constexpr int NRING = A1 - NT;
-    constexpr int MP2 = (A2 + NT - 1) / NT;  // phi, P positions per thread
-    constexpr int MP1 = (A1 + NT - 1) / NT;  // v positions per thread
     static_assert(NRING <= NT, "one ring position per thread at most");
-    // load slots: own cell + one apron ring, where the ring fits one slot
-    constexpr bool RING2 = MP2 == 2 && A2 - NT <= NT;
-    constexpr bool RING1 = MP1 == 2 && A1 - NT <= NT;
+    // apron rings around the tile and the loads per thread that cover them
+    // as (position, field) pairs
+    constexpr int R2 = A2 - NT, R1 = A1 - NT;
+    constexpr int NR2 = (4 * R2 + NT - 1) / NT, NR1 = (3 * R1 + NT - 1) / NT;
     // per position: phi, Px, Py, Pz x (f, dz, sz) | vx, vy, vz x (f, dz, sz) |
     // Sigma(9, [3a+b]), mu(3), mu0 of one plane
     constexpr int SP = 13, SV = 9, SS = 13;
@@ -155,68 +154,63 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
         const bool own = (gx < N) && (gy < N);
         const unsigned gown = 4u * ((unsigned)gy * (unsigned)N + (unsigned)gx);
 
-        // in-plane byte offsets of this thread's phi/P and v positions, and
-        // their LDS indices (-1: a clamped duplicate, not stored).  Where a
-        // ring fits one slot, slot 0 is the own cell (a wave loads whole
-        // aligned rows) and slot 1 the apron ring around the tile;
-        // otherwise positions follow the LDS order.
-        unsigned off2[MP2], off1[MP1];
-        int lds2[MP2], lds1[MP1];
+        // Loads of a plane: the thread's own cell (a wave loads whole aligned
+        // rows), one per field, and the apron ring around the tile as
+        // (position, field) pairs spread over NR2 (phi, P) and NR1 (v) loads
+        // per thread.  Ring loads address the fields from the lowest field
+        // pointer of their group (one allocation: 32-bit offsets).
+        gptr<const float> gb2 = gin[0], gb1 = gv[0];
 #pragma unroll
-        for (int k = 0; k < MP2; ++k) {
-            int i = min(tid + k * NT, A2 - 1), l = tid + k * NT < A2 ? i : -1;
-            if constexpr (RING2) {
-                if (k == 0) {
-                    i = l = o2c;
-                } else {
-                    const int r = min(tid, A2 - NT - 1);
-                    if (r < 2 * W2) i = r;                                                  // rows y0-2, y0-1
-                    else if (r < 4 * W2) i = (TY + 2) * W2 + (r - 2 * W2);                  // rows y0+TY, +TY+1
-                    else if (r < 4 * W2 + 2 * TY) i = (2 + (r - 4 * W2) / 2) * W2 + (r - 4 * W2) % 2;
-                    else i = (2 + (r - 4 * W2 - 2 * TY) / 2) * W2 + W2 - 2 + (r - 4 * W2 - 2 * TY) % 2;
-                    l = tid < A2 - NT ? i : -1;
-                }
-            }
-            lds2[k] = l;
-            off2[k] = 4u * ((unsigned)wrapi(y0 - 2 + i / W2, N) * (unsigned)N + (unsigned)wrapi(x0 - 2 + i % W2, N));
+        for (int f = 1; f < 4; ++f) gb2 = gin[f] < gb2 ? gin[f] : gb2;
+#pragma unroll
+        for (int f = 1; f < 3; ++f) gb1 = gv[f] < gb1 ? gv[f] : gb1;
+        const unsigned own2 = 4u * ((unsigned)wrapi(gy, N) * (unsigned)N + (unsigned)wrapi(gx, N));
+        const unsigned own1 = own2;
+        unsigned roff2[NR2], roff1[NR1];
+        int rl2[NR2], rl1[NR1];  // LDS float index of the pair's f array (-1: past the last pair)
+#pragma unroll
+        for (int m = 0; m < NR2; ++m) {
+            const int j = min(tid + m * NT, 4 * R2 - 1), f = j / R2, r = j % R2;
+            int i;
+            if (r < 2 * W2) i = r;                                                  // rows y0-2, y0-1
+            else if (r < 4 * W2) i = (TY + 2) * W2 + (r - 2 * W2);                  // rows y0+TY, +TY+1
+            else if (r < 4 * W2 + 2 * TY) i = (2 + (r - 4 * W2) / 2) * W2 + (r - 4 * W2) % 2;
+            else i = (2 + (r - 4 * W2 - 2 * TY) / 2) * W2 + W2 - 2 + (r - 4 * W2 - 2 * TY) % 2;
+            roff2[m] = 4u * (unsigned)(gin[f] - gb2)
+                     + 4u * ((unsigned)wrapi(y0 - 2 + i / W2, N) * (unsigned)N + (unsigned)wrapi(x0 - 2 + i % W2, N));
+            rl2[m] = tid + m * NT < 4 * R2 ? i * SP + f * NZ : -1;
         }
 #pragma unroll
-        for (int k = 0; k < MP1; ++k) {
-            int i = min(tid + k * NT, A1 - 1), l = tid + k * NT < A1 ? i : -1;
-            if constexpr (RING1) {
-                if (k == 0) {
-                    i = l = o1c;
-                } else {
-                    const int r = min(tid, A1 - NT - 1);
-                    if (r < W1) i = r;                                       // row y0-1
-                    else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);       // row y0+TY
-                    else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;     // column x0-1
-                    else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);          // column x0+TX
-                    l = tid < A1 - NT ? i : -1;
-                }
-            }
-            lds1[k] = l;
-            off1[k] = 4u * ((unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N));
+        for (int m = 0; m < NR1; ++m) {
+            const int j = min(tid + m * NT, 3 * R1 - 1), f = j / R1, r = j % R1;
+            int i;
+            if (r < W1) i = r;                                       // row y0-1
+            else if (r < 2 * W1) i = (TY + 1) * W1 + (r - W1);       // row y0+TY
+            else if (r < 2 * W1 + TY) i = (r - 2 * W1 + 1) * W1;     // column x0-1
+            else i = (r - 2 * W1 - TY + 1) * W1 + (W1 - 1);          // column x0+TX
+            roff1[m] = 4u * (unsigned)(gv[f] - gb1)
+                     + 4u * ((unsigned)wrapi(y0 - 1 + i / W1, N) * (unsigned)N + (unsigned)wrapi(x0 - 1 + i % W1, N));
+            rl1[m] = tid + m * NT < 3 * R1 ? i * SV + f * NZ : -1;
         }
-        // three z-planes per position in registers, in slots rotating with the
+        // three z-planes per value in registers, in slots rotating with the
         // z-march: in phase K the slots K, K+1, K+2 (mod 3) hold planes zi-2,
         // zi-1, zi
-        float in[MP2][4][3];
-        float vin[MP1][3][3];
-        auto load_plane = [&](int slot, int z, const unsigned* p2, const unsigned* p1) {
+        float in[4][3], inr[NR2][3];
+        float vin[3][3], vinr[NR1][3];
+        auto load_plane = [&](int slot, int z, unsigned o2, const unsigned* r2, unsigned o1, const unsigned* r1) {
             const size_t zo = (size_t)wrapi(z, N) * NN;
 #pragma unroll
-            for (int k = 0; k < MP2; ++k)
+            for (int f = 0; f < 4; ++f) in[f][slot] = ldb(gin[f] + zo, o2);
 #pragma unroll
-                for (int f = 0; f < 4; ++f) in[k][f][slot] = ldb(gin[f] + zo, p2[k]);
+            for (int m = 0; m < NR2; ++m) inr[m][slot] = ldb(gb2 + zo, r2[m]);
 #pragma unroll
-            for (int k = 0; k < MP1; ++k)
+            for (int f = 0; f < 3; ++f) vin[f][slot] = ldb(gv[f] + zo, o1);
 #pragma unroll
-                for (int f = 0; f < 3; ++f) vin[k][f][slot] = ldb(gv[f] + zo, p1[k]);
+            for (int m = 0; m < NR1; ++m) vinr[m][slot] = ldb(gb1 + zo, r1[m]);
         };
-        load_plane(0, zb - 2, off2, off1);
-        load_plane(1, zb - 1, off2, off1);
-        load_plane(2, zb, off2, off1);
+        load_plane(0, zb - 2, own2, roff2, own1, roff1);
+        load_plane(1, zb - 1, own2, roff2, own1, roff1);
+        load_plane(2, zb, own2, roff2, own1, roff1);
         wait_vmem();
 
         // D.Sigma, L mu, L mu0 accumulated per output plane, slot-rotated like
@@ -245,60 +239,58 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             ti &= 0x7fff;  // known non-negative again (a no-op)
             // likewise the in-plane global byte offsets, so that loads and
             // stores take a scalar plane base plus this 32-bit offset
-            unsigned go = gown, q2[MP2], q1[MP1];
-            asm volatile("" : "+v"(go));
+            unsigned go = gown, o2 = own2, o1 = own1, q2[NR2], q1[NR1];
+            asm volatile("" : "+v"(go), "+v"(o2), "+v"(o1));
 #pragma unroll
-            for (int k = 0; k < MP2; ++k) {
-                q2[k] = off2[k];
-                asm volatile("" : "+v"(q2[k]));
+            for (int m = 0; m < NR2; ++m) {
+                q2[m] = roff2[m];
+                asm volatile("" : "+v"(q2[m]));
             }
 #pragma unroll
-            for (int k = 0; k < MP1; ++k) {
-                q1[k] = off1[k];
-                asm volatile("" : "+v"(q1[k]));
+            for (int m = 0; m < NR1; ++m) {
+                q1[m] = roff1[m];
+                asm volatile("" : "+v"(q1[m]));
             }
             // ---- 1. z-prepare plane zm = zi-1 ------------------------------
             __syncthreads();
+            // f, f(z+1)-f(z-1), S_z f of one value into LDS at float index q
+            auto zprep = [&](float* q, const float* t) {
+                const float a = t[S0], b = t[S1], c = t[S2];
+                q[ZF] = b;
+                q[ZD] = c - a;
+                q[ZS] = co.sw0 * b + co.sw1 * (a + c);
+            };
+            {
+                int i2 = o2c * SP, i1 = o1c * SV;
+                asm volatile("" : "+v"(i2), "+v"(i1));
 #pragma unroll
-            for (int k = 0; k < MP2; ++k) {
-                int i = lds2[k];
-                asm volatile("" : "+v"(i));
-                if (i >= 0) {
+                for (int f = 0; f < 4; ++f) zprep(zp + i2 + f * NZ, in[f]);
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) {
-                        const float a = in[k][f][S0], b = in[k][f][S1], c = in[k][f][S2];
-                        float* q = zp + i * SP + f * NZ;
-                        q[ZF] = b;
-                        q[ZD] = c - a;
-                        q[ZS] = co.sw0 * b + co.sw1 * (a + c);
-                    }
-                }
+                for (int f = 0; f < 3; ++f) zprep(zv + i1 + f * NZ, vin[f]);
             }
 #pragma unroll
-            for (int k = 0; k < MP1; ++k) {
-                int i = lds1[k];
+            for (int m = 0; m < NR2; ++m) {
+                int i = rl2[m];
                 asm volatile("" : "+v"(i));
-                if (i >= 0) {
+                if (i >= 0) zprep(zp + i, inr[m]);
+            }
 #pragma unroll
-                    for (int f = 0; f < 3; ++f) {
-                        const float a = vin[k][f][S0], b = vin[k][f][S1], c = vin[k][f][S2];
-                        float* q = zv + i * SV + f * NZ;
-                        q[ZF] = b;
-                        q[ZD] = c - a;
-                        q[ZS] = co.sw0 * b + co.sw1 * (a + c);
-                    }
-                }
+            for (int m = 0; m < NR1; ++m) {
+                int i = rl1[m];
+                asm volatile("" : "+v"(i));
+                if (i >= 0) zprep(zv + i, vinr[m]);
             }
             // roll the planes and prefetch plane zi+1 into the newest slot
+            auto roll = [](float* t) { t[0] = t[1]; t[1] = t[2]; };
 #pragma unroll
-            for (int k = 0; k < MP2; ++k)
+            for (int f = 0; f < 4; ++f) roll(in[f]);
 #pragma unroll
-                for (int f = 0; f < 4; ++f) { in[k][f][0] = in[k][f][1]; in[k][f][1] = in[k][f][2]; }
+            for (int m = 0; m < NR2; ++m) roll(inr[m]);
 #pragma unroll
-            for (int k = 0; k < MP1; ++k)
+            for (int f = 0; f < 3; ++f) roll(vin[f]);
 #pragma unroll
-                for (int f = 0; f < 3; ++f) { vin[k][f][0] = vin[k][f][1]; vin[k][f][1] = vin[k][f][2]; }
-            if (zi <= ze) load_plane(2, zi + 1, q2, q1);
+            for (int m = 0; m < NR1; ++m) roll(vinr[m]);
+            if (zi <= ze) load_plane(2, zi + 1, o2, q2, o1, q1);
             __syncthreads();
 
             // ---- 2. intermediates of plane zm over the 1-cell apron --------
