@@ -16,6 +16,11 @@
 // 16) are composed from them with compile-time twiddles.  Sign convention:
 // forward e^{-2 pi i jk/N}, inverse e^{+2 pi i jk/N}, both unnormalised, as
 // cufftExecZ2Z.
+//
+// The first forward pass (x, or the fused x+y) divides the imaginary part of
+// s by the droplet's power-of-two sigma and the z pass takes the k = 0
+// coefficient from the FP64 sum of s; both come from k_proj_stats
+// (Drop::sigma, Drop::stat).
 #include "ttsh_dev.h"
 #include "ttsh_kernels.h"
 
