@@ -347,8 +347,53 @@ int ttsh_init_droplet(int N, double R, double xi, int pmode, uint64_t seed, doub
 
 namespace ttsh {
 
-// Format a double as std::fixed << setprecision(15) does (printf "%.15f").
-static inline int fmt15(char* out, double v) { return std::snprintf(out, 64, "%.15f", v); }
+// An FP32 value as std::fixed << setprecision(15) prints the double it
+// converts to (printf "%.15f": the exact binary value rounded half to even
+// at the 15th decimal).  The value is M * 2^e with M < 2^24, so
+// |v| * 10^15 = M * 10^15 * 2^e is rounded exactly in 128-bit integers and
+// printed digit by digit; NaN, infinities and |v| >= 2^40 go to snprintf.
+static int fmt15(char* out, float v)
+{
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    const int ex = (int)((b >> 23) & 0xff);
+    if (ex == 0xff || ex >= 127 + 40) return std::snprintf(out, 64, "%.15f", (double)v);
+    const uint64_t M = ex ? ((b & 0x7fffffu) | 0x800000u) : (b & 0x7fffffu);
+    const int e = ex ? ex - 150 : -149;  // v = +-M * 2^e
+    constexpr uint64_t P15 = 1000000000000000ull;
+    unsigned __int128 q;
+    if (e >= 0) {
+        q = ((unsigned __int128)M * P15) << e;  // < 2^90 for e < 17
+    } else {
+        const unsigned __int128 num = (unsigned __int128)M * P15;  // < 2^74
+        const int sh = -e;
+        if (sh > 76) {
+            q = 0;  // below half a unit of the 15th decimal
+        } else {
+            q = num >> sh;
+            const unsigned __int128 rem = num - (q << sh), half = (unsigned __int128)1 << (sh - 1);
+            if (rem > half || (rem == half && (q & 1))) ++q;
+        }
+    }
+    uint64_t ip = (uint64_t)(q / P15), fp = (uint64_t)(q % P15);
+    char* o = out;
+    if (b >> 31) *o++ = '-';
+    char buf[24];
+    int nd = 0;
+    do {
+        buf[nd++] = (char)('0' + ip % 10);
+        ip /= 10;
+    } while (ip);
+    while (nd) *o++ = buf[--nd];
+    *o++ = '.';
+    for (int d = 14; d >= 0; --d) {
+        o[d] = (char)('0' + fp % 10);
+        fp /= 10;
+    }
+    o += 15;
+    *o = '\0';
+    return (int)(o - out);
+}
 
 // Text of one ExpoConf row i (y): the N*N lines of x, then z, in order.
 static void format_row(std::string& out, int N, int expo_data, int i, const float* phi, const float* P,
@@ -363,14 +408,14 @@ static void format_row(std::string& out, int N, int expo_data, int i, const floa
             if (expo_data == 1) {
                 const float vals[7] = {v[idx], v[n + idx], v[2 * n + idx], P[idx], P[n + idx], P[2 * n + idx], phi[idx]};
                 for (int q = 0; q < 7; ++q) {
-                    const int m = fmt15(tmp, (double)vals[q]);
+                    const int m = fmt15(tmp, vals[q]);
                     out.append(tmp, m);
                     out.push_back(q < 6 ? ' ' : '\n');
                 }
             } else if (expo_data == -1) {
                 const float vals[2] = {v[idx], v[n + idx]};
                 for (int q = 0; q < 2; ++q) {
-                    const int m = fmt15(tmp, (double)vals[q]);
+                    const int m = fmt15(tmp, vals[q]);
                     out.append(tmp, m);
                     out.push_back(' ');
                 }

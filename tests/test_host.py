@@ -195,6 +195,40 @@ def test_conf_format(ttsh, tmp_path):
         assert np.abs(got - want).max() < 1e-15 + 1e-15 * np.abs(want).max()
 
 
+def test_conf_text_is_printf_exact(ttsh, tmp_path):
+    """Every value of a snapshot is the "%.15f" text of the FP32 value as a
+    double (TTSH3D.cu:983-998: std::fixed, setprecision(15)), byte for byte:
+    random finite bit patterns over the whole exponent range, exact halfway
+    cases at the 15th decimal (odd multiples of 2^-16 round half to even),
+    signed zeros, subnormals, the 2^40 boundary of the integer path, and
+    non-finite values."""
+    N = 16
+    rng = np.random.default_rng(11)
+    n = 7 * N ** 3
+    bits = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    vals = bits.view(np.float32).copy()
+    vals[~np.isfinite(vals)] = 1.5
+    special = [0.0, -0.0, 2.0 ** -16, 3 * 2.0 ** -16, -5 * 2.0 ** -16, 2.0 ** -17, 1e-45, -1e-45, 1.17549435e-38,
+               0.5e-15, 1.5e-15, 2.5e-15, 1.0, -1.0, 123.456, 2.0 ** 39, 2.0 ** 40 - 2.0 ** 16, 2.0 ** 40, -2.0 ** 41,
+               3.4e38, np.inf, -np.inf, np.nan, 0.1, 0.7, 0.999999999, 33554431.0]
+    vals[:len(special)] = np.array(special, np.float32)
+    # halfway cases k * 2^-16 for odd k, and small multiples of 2^-17..2^-20
+    m = 2000
+    vals[100:100 + m] = (rng.integers(-(2 ** 20), 2 ** 20, m) * 2 + 1).astype(np.float32) * np.float32(2.0 ** -16)
+    vals[3000:3000 + m] = rng.integers(-(2 ** 20), 2 ** 20, m).astype(np.float32) * np.float32(2.0 ** -19)
+    cube = vals.reshape(7, N, N, N)
+    v, P, phi = cube[0:3], cube[3:6], cube[6]
+    path = str(tmp_path / "conf_x.dat")
+    ttsh.write_conf_arrays(path, phi, P, v)
+    got = open(path).read().split()
+    # the row order of ExpoConf: y outer, x, z inner; fields vx vy vz px py pz phi
+    order = np.stack([v[0], v[1], v[2], P[0], P[1], P[2], phi])  # [f, z, y, x]
+    want = ["%.15f" % float(x) for x in np.transpose(order, (2, 3, 1, 0)).reshape(-1)]
+    assert len(got) == len(want)
+    bad = [(g, w) for g, w in zip(got, want) if g != w]
+    assert not bad, bad[:5]
+
+
 def test_conf_format_expo_minus_one(ttsh, tmp_path):
     N = 32
     z = np.zeros((N, N, N), np.float32)
