@@ -396,32 +396,38 @@ static int fmt15(char* out, float v)
 }
 
 // Text of one ExpoConf row i (y): the N*N lines of x, then z, in order.
-static void format_row(std::string& out, int N, int expo_data, int i, const float* phi, const float* P,
-                       const float* v)
+// Returns the row's length in out (the buffer keeps its size between rows).
+static size_t format_row(std::string& out, int N, int expo_data, int i, const float* phi, const float* P,
+                         const float* v)
 {
     const size_t n = (size_t)N * N * N;
-    char tmp[64];
-    out.clear();
+    // a value takes at most 64 bytes (snprintf's NaN/inf/large-value text);
+    // the buffer grows only if a line might not fit
+    constexpr size_t kLine = 7 * 64;
+    if (out.size() < (size_t)N * N * 7 * 20 + kLine) out.resize((size_t)N * N * 7 * 20 + kLine);
+    size_t pos = 0;
     for (int j = 0; j < N; ++j) {
         for (int k = 0; k < N; ++k) {
+            if (out.size() - pos < kLine) out.resize(2 * out.size());
+            char* o = &out[pos];
             const size_t idx = ((size_t)k * N + i) * N + j;
             if (expo_data == 1) {
                 const float vals[7] = {v[idx], v[n + idx], v[2 * n + idx], P[idx], P[n + idx], P[2 * n + idx], phi[idx]};
                 for (int q = 0; q < 7; ++q) {
-                    const int m = fmt15(tmp, vals[q]);
-                    out.append(tmp, m);
-                    out.push_back(q < 6 ? ' ' : '\n');
+                    o += fmt15(o, vals[q]);
+                    *o++ = q < 6 ? ' ' : '\n';
                 }
             } else if (expo_data == -1) {
                 const float vals[2] = {v[idx], v[n + idx]};
                 for (int q = 0; q < 2; ++q) {
-                    const int m = fmt15(tmp, vals[q]);
-                    out.append(tmp, m);
-                    out.push_back(' ');
+                    o += fmt15(o, vals[q]);
+                    *o++ = ' ';
                 }
             }
+            pos = (size_t)(o - &out[0]);
         }
     }
+    return pos;
 }
 
 int write_conf_host(const char* path, int N, int expo_data, const float* phi, const float* P, const float* v)
@@ -433,15 +439,16 @@ int write_conf_host(const char* path, int N, int expo_data, const float* phi, co
     // order.
     const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     std::vector<std::string> rows(nt);
+    std::vector<size_t> len(nt, 0);
     for (int i0 = 0; i0 < N; i0 += nt) {
         const int cnt = std::min(nt, N - i0);
         std::vector<std::thread> th;
         for (int t = 1; t < cnt; ++t)
-            th.emplace_back([&, t] { format_row(rows[t], N, expo_data, i0 + t, phi, P, v); });
-        format_row(rows[0], N, expo_data, i0, phi, P, v);
+            th.emplace_back([&, t] { len[t] = format_row(rows[t], N, expo_data, i0 + t, phi, P, v); });
+        len[0] = format_row(rows[0], N, expo_data, i0, phi, P, v);
         for (auto& x : th) x.join();
         for (int t = 0; t < cnt; ++t) {
-            if (!rows[t].empty() && std::fwrite(rows[t].data(), 1, rows[t].size(), f) != rows[t].size()) {
+            if (len[t] && std::fwrite(rows[t].data(), 1, len[t], f) != len[t]) {
                 std::fclose(f);
                 return fail(TTSH_EIO, std::string("write failed: ") + path);
             }

@@ -71,6 +71,7 @@ int main(int argc, char** argv)
     const std::string in = dir + "/input.dat";
     if (ttsh_write_input_dat(in.c_str(), &p)) return 3;
     ttsh_params q;
+    ttsh_params_default(&q);  // parsing keeps the fields input.dat does not hold
     if (ttsh_parse_input_dat(in.c_str(), &q)) return 4;
     if (ttsh_validate_params(&q)) return 5;
     if (ttsh_parse_input_dat((dir + "/missing.dat").c_str(), &q) == 0) return 6;
