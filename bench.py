@@ -143,7 +143,8 @@ def batch_sweep(ttsh, steps, seed=1004, ndrop=64):
             "batched_sims_per_hour": round(sph_b, 2), "plain_sims_per_hour": round(sph_p, 2),
             "ratio": round(sph_b / sph_p, 3),
             "batched_mcell_steps_per_s": round(cells * steps / tb / 1e6, 1),
-            "note": "sims/hour for 2,000-step droplets, extrapolated from the timed steps"}
+            "note": "sims/hour for 2,000-step droplets, " + (
+                "timed over all 2,000 steps" if steps == full else "extrapolated from the timed steps")}
 
 
 def main():
