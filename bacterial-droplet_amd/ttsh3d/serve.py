@@ -238,7 +238,7 @@ def main(argv=None):
     reqs = make_requests(a.requests, rate, tuple(a.sizes), tuple(a.steps), a.seed)
     be = GpuBackend(a.max_slots, budget)
     srv = Server(be, macro=a.macro)
-    done, wall = srv.run(reqs)
+    done, wall = srv.run(reqs, log=lambda m: print(m, flush=True))
     be.close()
     out = summarize(done, wall)
     out.update({"requests": a.requests, "rate_per_s": rate, "capacity_mcell_steps_per_s": round(cap / 1e6, 1),
