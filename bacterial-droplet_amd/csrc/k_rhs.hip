@@ -252,7 +252,10 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                 asm volatile("" : "+v"(q1[m]));
             }
             // ---- 1. z-prepare plane zm = zi-1 ------------------------------
-            __syncthreads();
+            // Two barriers per plane suffice: zp and zv are read only in stage
+            // 2, which every thread ends before the previous plane's barrier
+            // ahead of stage 3; sM is written in stage 2 and read in stage 3,
+            // and the barrier after stage 1 orders them across planes.
             // f, f(z+1)-f(z-1), S_z f of one value into LDS at float index q
             auto zprep = [&](float* q, const float* t) {
                 const float a = t[S0], b = t[S1], c = t[S2];
