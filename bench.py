@@ -26,7 +26,8 @@ plain copy kernel), when bench_traffic.json beside this file holds them for
 the grid size (profiles/step_<N>_hbm_traffic.md records those runs).
 
 cpu_baseline: the FP64 oracle (OpenMP, all host threads) on the same initial
-state, timed for a bounded number of steps.
+state, timed over whole steps until at least 10 s of CPU work (a bounded
+sample of the same workload).
 
 The batched-sweep line (--batch, on by default) runs 64 droplets with L drawn
 from {48, 64, 96, 128} (R = 0.3 L, cA = default x logU[0.1, 10], La^2 =
@@ -79,7 +80,7 @@ def workload(ttsh, N):
     return p, phi, P, v
 
 
-def cpu_baseline(p, phi, P, v, budget_s=15.0, max_steps=20):
+def cpu_baseline(p, phi, P, v, min_s=10.0, max_steps=10000):
     from oracle import oracle as O
     threads = os.cpu_count() or 1
     O.set_threads(threads)
@@ -87,11 +88,12 @@ def cpu_baseline(p, phi, P, v, budget_s=15.0, max_steps=20):
     o = O.Oracle(p.Nx, faithful=True, **{k: getattr(p, k) for k in keys})
     o.set_state(phi, P, v)
     o.step(1)  # warm-up
+    # whole steps until at least min_s of CPU work has been timed
     n, t0 = 0, time.perf_counter()
     while n < max_steps:
         o.step(1)
         n += 1
-        if time.perf_counter() - t0 > budget_s:
+        if time.perf_counter() - t0 >= min_s:
             break
     dt = time.perf_counter() - t0
     cells = p.Nx ** 3
