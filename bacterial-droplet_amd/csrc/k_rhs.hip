@@ -351,58 +351,48 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
 #pragma unroll
                 for (int j = 0; j < 7; ++j) asm volatile("" : "+v"(nxt[j]));
             }
-            // Intermediates of the apron ring.  When one pass of the tile's
-            // threads covers them, they are spread as (position, part) pairs
-            // (part 0: mu0, part 1+b: the intermediates of P_b) over every
-            // wave; otherwise each of the first NRING threads computes all 13
-            // of one position.  Both use the own cell's expressions.
-            if constexpr (4 * NRING <= NT) {
-                if (ti < 4 * NRING) {
-                    const int f = ti / NRING, r = ti % NRING;
-                    int lx1, ly1;
-                    if (r < W1) { lx1 = r; ly1 = 0; }
-                    else if (r < 2 * W1) { lx1 = r - W1; ly1 = H1 - 1; }
-                    else if (r < 2 * W1 + TY) { lx1 = 0; ly1 = 1 + (r - 2 * W1); }
-                    else { lx1 = W1 - 1; ly1 = 1 + (r - 2 * W1 - TY); }
-                    const int q2 = (ly1 + 1) * W2 + (lx1 + 1);
-                    const int q1 = ly1 * W1 + lx1;
-                    const Rows rq = rows_at<W2, SP>(zp, q2);
-                    const float ph = rq.c[SP + ZF];
-                    float* m = sM + q1 * SS;
-                    if (f == 0) {
-                        const float ph2 = ph * ph;
-                        m[12] = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * lap_rows<SP>(rq, 0, co);
-                    } else {
-                        const int b = f - 1, fo = f * NZ;
-                        const Der4 d = der_rows<SP>(rq, fo, co);
-                        m[9 + b] = ph * rq.c[SP + fo + ZF] + co.La2 * d.l;
-                        m[0 * 3 + b] = ph * d.dx;
-                        m[1 * 3 + b] = ph * d.dy;
-                        m[2 * 3 + b] = ph * d.dz;
-                    }
-                }
-            } else {
-                if (ti < NRING) {
-                    const int r = ti;
-                    int lx1, ly1;
-                    if (r < W1) { lx1 = r; ly1 = 0; }
-                    else if (r < 2 * W1) { lx1 = r - W1; ly1 = H1 - 1; }
-                    else if (r < 2 * W1 + TY) { lx1 = 0; ly1 = 1 + (r - 2 * W1); }
-                    else { lx1 = W1 - 1; ly1 = 1 + (r - 2 * W1 - TY); }
-                    const int q2 = (ly1 + 1) * W2 + (lx1 + 1);
-                    const int q1 = ly1 * W1 + lx1;
-                    const Rows rq = rows_at<W2, SP>(zp, q2);
-                    const float ph = rq.c[SP + ZF];
+            // Intermediates of the apron ring, spread over every wave as
+            // (position, part) pairs when one pass of the tile's threads covers
+            // them: four parts (mu0, then the intermediates of P_x, P_y, P_z),
+            // or three (those of P_b, with mu0 in part 0).  Otherwise each of
+            // the first NRING threads computes all 13 of one position.  All use
+            // the own cell's expressions.
+            constexpr int NPART = 4 * NRING <= NT ? 4 : 3 * NRING <= NT ? 3 : 1;
+            if (ti < NPART * NRING) {
+                const int f = ti / NRING, r = ti % NRING;
+                int lx1, ly1;
+                if (r < W1) { lx1 = r; ly1 = 0; }
+                else if (r < 2 * W1) { lx1 = r - W1; ly1 = H1 - 1; }
+                else if (r < 2 * W1 + TY) { lx1 = 0; ly1 = 1 + (r - 2 * W1); }
+                else { lx1 = W1 - 1; ly1 = 1 + (r - 2 * W1 - TY); }
+                const int q2 = (ly1 + 1) * W2 + (lx1 + 1);
+                const int q1 = ly1 * W1 + lx1;
+                const Rows rq = rows_at<W2, SP>(zp, q2);
+                const float ph = rq.c[SP + ZF];
+                float* m = sM + q1 * SS;
+                // intermediates of P_b at this position
+                auto pint = [&](int b, int fo) {
+                    const Der4 d = der_rows<SP>(rq, fo, co);
+                    m[9 + b] = ph * rq.c[SP + fo + ZF] + co.La2 * d.l;
+                    m[0 * 3 + b] = ph * d.dx;
+                    m[1 * 3 + b] = ph * d.dy;
+                    m[2 * 3 + b] = ph * d.dz;
+                };
+                auto mu0 = [&]() {
                     const float ph2 = ph * ph;
-                    float* m = sM + q1 * SS;
                     m[12] = co.aPhi2 * (ph - 3.f * ph2 + 2.f * ph2 * ph) - co.kPhi * lap_rows<SP>(rq, 0, co);
+                };
+                if constexpr (NPART == 4) {
+                    if (f == 0) mu0();
+                    else pint(f - 1, f * NZ);
+                } else if constexpr (NPART == 3) {
+                    if (f == 0) mu0();
+                    pint(f, (1 + f) * NZ);
+                } else {
+                    mu0();
 #pragma unroll
                     for (int b = 0; b < 3; ++b) {
-                        const Der4 d = der_rows<SP>(rq, (1 + b) * NZ, co);
-                        m[9 + b] = ph * rq.c[SP + (1 + b) * NZ + ZF] + co.La2 * d.l;
-                        m[0 * 3 + b] = ph * d.dx;
-                        m[1 * 3 + b] = ph * d.dy;
-                        m[2 * 3 + b] = ph * d.dz;
+                        pint(b, (1 + b) * NZ);
                         __builtin_amdgcn_sched_barrier(0);
                     }
                 }
