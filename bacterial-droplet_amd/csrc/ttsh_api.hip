@@ -104,6 +104,7 @@ struct Slot {
     int64_t limit = -1;     // max steps, -1 = none
     double T = 0.0;         // T accumulated in double as the reference (TTSH3D.cu:342)
     int rhs_tz = 16, proj_tz = 8;
+    bool rhs_tz_set = false;  // rhs_tz came from the parameters, not the tuning table
 };
 
 struct Table {
@@ -120,6 +121,7 @@ struct Engine {
     bool use_graph = false;
     int graph_steps = 16;
     int rhs_variant = -1;  // engine-wide K_rhs tile variant (-1: from the first droplet)
+    int rhs_active = -1;   // the variant the work tables and launches use (see rebuild)
     int cus = 256;
 
     std::vector<Slot> slots;
@@ -406,6 +408,7 @@ struct Engine {
         s.limit = -1;
         s.T = p.T0;
         s.rhs_tz = std::min(p.rhs_tz > 0 ? p.rhs_tz : (ttz > 0 ? ttz : 16), N);
+        s.rhs_tz_set = p.rhs_tz > 0;
         s.proj_tz = std::min(p.corr_tz > 0 ? p.corr_tz : (tpz > 0 ? tpz : 8), N);
         tables_dirty = true;
         return TTSH_OK;
@@ -443,12 +446,43 @@ struct Engine {
 
     int64_t remaining(const Slot& s) const { return s.limit < 0 ? INT64_MAX : std::max<int64_t>(0, s.limit - s.steps); }
 
+    // K_rhs layout for a batch.  The tuned tile and z-depth of a size are
+    // chosen for that droplet alone, with enough z-chunks to fill the device;
+    // each chunk re-reads two warm-up planes.  A batch that fills the device
+    // by itself runs K_rhs with the 32x8 tile (three groups per CU) and
+    // chunks of at least kBatchRhsTz planes, which cuts the warm-up share and
+    // gives items of nearly equal cost.  It applies when at least two
+    // droplets are active, none asks for a tile, every one fits 32x8, and the
+    // items make at least two rounds of resident groups.  K_rhs has no
+    // reductions, so a droplet's result does not depend on this layout.
+    static constexpr int kBatchRhsTz = 32;
+    bool batch_rhs_layout() const
+    {
+        int active = 0;
+        int64_t items = 0;
+        for (const Slot& s : slots) {
+            if (!s.used || remaining(s) <= 0) continue;
+            if (s.prm.rhs_variant >= 0 || !variant_fits(RHS_32x8, s.N)) return false;
+            ++active;
+            const int tz = std::min(std::max(s.rhs_tz, kBatchRhsTz), s.N);
+            items += (int64_t)((s.N + 31) / 32) * ((s.N + 7) / 8) * ((s.N + tz - 1) / tz);
+        }
+        return active >= 2 && items >= 2 * 3 * (int64_t)cus;
+    }
+
     // Rebuild descriptor and work tables from the active slots.
     int rebuild(int steps_this_call)
     {
         std::vector<Work> w[kNTables];
+        const bool batch = batch_rhs_layout();
+        const int rv = batch ? (int)RHS_32x8 : (rhs_variant < 0 ? 0 : rhs_variant);
+        if (rv != rhs_active) {
+            HIPCHK(hipStreamSynchronize(st));
+            rhs_active = rv;
+            drop_graphs();  // captured launches name the tile's kernel
+        }
         int tx, ty;
-        rhs_tile(rhs_variant < 0 ? 0 : rhs_variant, &tx, &ty);
+        rhs_tile(rhs_active, &tx, &ty);
         for (int si = 0; si < (int)slots.size(); ++si) {
             Slot& s = slots[si];
             h_drops[si] = s.drop;
@@ -458,7 +492,8 @@ struct Engine {
             if (rem <= 0) continue;
             const int N = s.N;
             // z is split into ceil(N/tz) chunks of near-equal depth
-            const int nzr = (N + s.rhs_tz - 1) / s.rhs_tz;
+            const int rtz = batch && !s.rhs_tz_set ? std::min(std::max(s.rhs_tz, kBatchRhsTz), N) : s.rhs_tz;
+            const int nzr = (N + rtz - 1) / rtz;
             for (int k = 0; k < nzr; ++k) {
                 const int zb = k * N / nzr, ze = (k + 1) * N / nzr;
                 for (int by = 0; by < (N + ty - 1) / ty; ++by)
@@ -533,7 +568,7 @@ struct Engine {
         auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
         const int* cnt = d_counts;
         HIPCHK(mark(KK_RHS));
-        HIPCHK(launch_rhs(rhs_variant, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
+        HIPCHK(launch_rhs(rhs_active, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
         HIPCHK(mark(KK_DIV));
         HIPCHK(launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
         HIPCHK(launch_proj_stats(d_drops, tab[T_STAT].d, cnt + T_STAT, kstep, grid(T_STAT), st));

@@ -106,3 +106,28 @@ def test_sweep_equals_runs_alone(ttsh, tmp_path):
         for f in sorted(os.listdir(d)):
             assert open(d / f, "rb").read() == open(tmp_path / "sweep" / name / f, "rb").read(), (name, f)
         assert os.path.exists(tmp_path / "sweep" / name / "input.dat")
+
+
+def test_device_filling_batch_layout_equals_alone(ttsh):
+    """Six 128^3 and two 96^3 droplets make 1,752 K_rhs items at the batch
+    layout (32x8 tiles, z-chunks of 32), more than two rounds of three groups
+    on 256 CUs, so the engine runs K_rhs that way.  Alone, each droplet runs
+    its tuned 32x16 tile with chunks of 8 or 16 planes.  K_rhs has no
+    reductions, so the results are bitwise equal."""
+    log("[device_filling_batch_layout_equals_alone]")
+    steps = 6
+    cases = [(128, -1.0 * (1 + 0.3 * i), 1.0 + 0.1 * i, 700 + i) for i in range(6)]
+    cases += [(96, -1.5, 1.2, 710), (96, -0.7, 0.9, 711)]
+    f = lambda a: a.astype(np.float32)
+    with ttsh.Batch(max_slots=len(cases), use_graph=True) as b:
+        slots = []
+        for N, cA, La, seed in cases:
+            p = ttsh.default_params(N=N, dt0=0.01, cA=cA, La=La)
+            phi, P, v = ttsh.init_droplet(N, 0.3 * N, 2.0, 0, seed)
+            slots.append(b.admit(p, f(phi), f(P), f(v), dtype=ttsh.F32))
+        b.step(steps)
+        got = [b.get_state(s) for s in slots]
+    for (N, cA, La, seed), g in zip(cases, got):
+        ref = alone(ttsh, N, cA, La, seed, steps)
+        for a, c in zip(ref, g):
+            assert np.array_equal(a, c), f"{N}^3 droplet seed {seed} differs between batch and alone"
