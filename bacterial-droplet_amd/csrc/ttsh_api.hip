@@ -53,7 +53,7 @@ static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_xy", "K_ff
 // Work tables: rhs, div/corr (shared), one item per droplet (the projection's
 // sums), and per grid size the x/y-pass table, the z-pass table and the
 // z-plane table of the fused x+y pass.
-enum { T_RHS = 0, T_PROJ = 1, T_STAT = 2, T_FFT0 = 3 };
+enum { T_RHS = 0, T_PROJ = 1, T_STAT = 2, T_CORR = 3, T_FFT0 = 4 };
 constexpr int kNTables = T_FFT0 + 3 * kNumSizes;
 static inline int t_xy(int s) { return T_FFT0 + 3 * s; }      // line groups of x or y passes
 static inline int t_z(int s) { return T_FFT0 + 3 * s + 1; }   // line groups of the z pass
@@ -456,6 +456,11 @@ struct Engine {
     // items make at least two rounds of resident groups.  K_rhs has no
     // reductions, so a droplet's result does not depend on this layout.
     static constexpr int kBatchRhsTz = 32;
+    // K_corr (no reductions either) takes chunks of at least this depth in
+    // the same batches.  K_div keeps the tuned depth: its per-item sums set
+    // the k = 0 coefficient and sigma, and their rounding depends on the
+    // chunks, so a droplet's chunks stay the same alone and in a batch.
+    static constexpr int kBatchCorrTz = 32;
     bool batch_rhs_layout() const
     {
         int active = 0;
@@ -506,6 +511,12 @@ struct Engine {
             for (int k = 0; k < nzp; ++k) {
                 const int zb = k * N / nzp, ze = (k + 1) * N / nzp;
                 for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), ord++});
+            }
+            const int ctz = batch && s.prm.corr_tz <= 0 ? std::min(std::max(s.proj_tz, kBatchCorrTz), N) : s.proj_tz;
+            const int nzc = (N + ctz - 1) / ctz;
+            for (int k = 0; k < nzc; ++k) {
+                const int zb = k * N / nzc, ze = (k + 1) * N / nzc;
+                for (int t = 0; t < ntile; ++t) w[T_CORR].push_back(Work{si, t, zb | (ze << 16), 0});
             }
             // k_proj_stats sums the divergence items' FP64 partial sums
             h_drops[si].npart = ord;
@@ -592,7 +603,7 @@ struct Engine {
             }
         }
         HIPCHK(mark(KK_CORR));
-        HIPCHK(launch_corr(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
+        HIPCHK(launch_corr(d_drops, tab[T_CORR].d, cnt + T_CORR, parity, kstep, grid(T_CORR), st));
         HIPCHK(mark(KK_N));
         return TTSH_OK;
     }

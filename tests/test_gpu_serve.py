@@ -111,9 +111,10 @@ def test_sweep_equals_runs_alone(ttsh, tmp_path):
 def test_device_filling_batch_layout_equals_alone(ttsh):
     """Six 128^3 and two 96^3 droplets make 1,752 K_rhs items at the batch
     layout (32x8 tiles, z-chunks of 32), more than two rounds of three groups
-    on 256 CUs, so the engine runs K_rhs that way.  Alone, each droplet runs
-    its tuned 32x16 tile with chunks of 8 or 16 planes.  K_rhs has no
-    reductions, so the results are bitwise equal."""
+    on 256 CUs, so the engine runs K_rhs that way and K_corr with chunks of 32.
+    Alone, each droplet runs its tuned 32x16 tile with chunks of 8 or 16
+    planes and K_corr with 8 or 11.  Neither kernel reduces anything, so the
+    results are bitwise equal."""
     log("[device_filling_batch_layout_equals_alone]")
     steps = 6
     cases = [(128, -1.0 * (1 + 0.3 * i), 1.0 + 0.1 * i, 700 + i) for i in range(6)]
