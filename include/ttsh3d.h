@@ -60,7 +60,12 @@ typedef struct ttsh_params {
                               stale halo does (TTSH3D.cu:366-371,405-407,665-667) */
     int32_t rhs_variant;   /* K_rhs tile: -1 = tuned default, else 0..6 */
     int32_t rhs_tz;        /* K_rhs z-range per work item, 0 = tuned default */
-    int32_t corr_tz;       /* corrector z-range per work item, 0 = tuned default */
+    int32_t corr_tz;       /* divergence and corrector z-range per work item,
+                              0 = tuned default.  With the defaults, a batch
+                              whose droplets fill the device runs K_rhs with
+                              the 32x8 tile and ranges of at least 32, and the
+                              corrector with ranges of at least 32; results do
+                              not change. */
 } ttsh_params;
 
 typedef struct ttsh_sim ttsh_sim;
