@@ -167,12 +167,49 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
                 for (int m = 0; m < 2; ++m) pre[slot][6 + m] = ldb(gb + zo, ring_w[m]);
             }
         };
+        // The sums are written per reduction block (see Drop::part): the
+        // item starts at block wk.c, and at the last plane of each block the
+        // work-group reduces them in a fixed order and starts the next.  The
+        // sums of a block are then the same whichever item covers it.
+        const int ntile = nbx * ((N + TY - 1) / TY);
+        int ordb = wk.c;
+        int zend = (wk.c / ntile + 1) * N / D.nzp;
+        auto flush = [&]() {
+            q[2] = q2;
+            q[3] = q3;
+            q[4] = q4;
+            q[5] = q5;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                for (int e = 0; e < kPartSums; ++e) q[e] += __shfl_down(q[e], off, 64);
+            }
+            __syncthreads();
+            if ((tid & 63) == 0) {
+#pragma unroll
+                for (int e = 0; e < kPartSums; ++e) red[kPartSums * (tid >> 6) + e] = q[e];
+            }
+            __syncthreads();
+            if (tid < kPartSums) {
+                double a = 0.0;
+                for (int i = 0; i < NT / 64; ++i) a += red[kPartSums * i + tid];
+                glb(D.part)[(size_t)kPartSums * ordb + tid] = a;
+            }
+#pragma unroll
+            for (int e = 0; e < kPartSums; ++e) q[e] = 0.0;
+            q2 = q3 = q4 = q5 = 0.f;
+            ordb += ntile;
+            zend = (ordb / ntile + 1) * N / D.nzp;
+        };
         auto body = [&](auto ph, int zi) {
             constexpr int S = decltype(ph)::value;
-            if (zi >= zb && zi < ze && own_in) {
+            // squares of this plane's inputs, added after the block check below
+            float sq4 = 0.f, sq5 = 0.f;
+            const bool sq = zi >= zb && zi < ze && own_in;
+            if (sq) {
                 const float* u = pre[S];
-                q4 += u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-                q5 += u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
+                sq4 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                sq5 = u[3] * u[3] + u[4] * u[4] + u[5] * u[5];
             }
             __syncthreads();
             {
@@ -213,6 +250,11 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
                 q2 += sv.x * sv.x;
                 q3 += sv.y * sv.y;
             }
+            if (zo >= zb && zo == zend - 1) flush();  // uniform across the work-group
+            if (sq) {
+                q4 += sq4;
+                q5 += sq5;
+            }
 #pragma unroll
             for (int F = 0; F < 2; ++F) { am[F] = ac[F]; ac[F] = ap[F]; ap[F] = 0.f; }
         };
@@ -225,27 +267,7 @@ __global__ __launch_bounds__(TX * TY) void k_div(const Drop* __restrict__ drops,
             body(Ph<1>{}, zi);
             if (++zi > ze) break;
         }
-        // the item's sums, reduced in a fixed order
-        q[2] = q2;
-        q[3] = q3;
-        q[4] = q4;
-        q[5] = q5;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-            for (int e = 0; e < kPartSums; ++e) q[e] += __shfl_down(q[e], off, 64);
-        }
-        __syncthreads();
-        if ((tid & 63) == 0) {
-#pragma unroll
-            for (int e = 0; e < kPartSums; ++e) red[kPartSums * (tid >> 6) + e] = q[e];
-        }
-        __syncthreads();
-        if (tid < kPartSums) {
-            double a = 0.0;
-            for (int i = 0; i < NT / 64; ++i) a += red[kPartSums * i + tid];
-            glb(D.part)[(size_t)kPartSums * wk.c + tid] = a;
-        }
+        // the item ends at a block end, so its last block was written above
     }
 }
 

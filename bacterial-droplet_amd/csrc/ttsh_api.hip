@@ -456,10 +456,9 @@ struct Engine {
     // items make at least two rounds of resident groups.  K_rhs has no
     // reductions, so a droplet's result does not depend on this layout.
     static constexpr int kBatchRhsTz = 32;
-    // K_corr (no reductions either) takes chunks of at least this depth in
-    // the same batches.  K_div keeps the tuned depth: its per-item sums set
-    // the k = 0 coefficient and sigma, and their rounding depends on the
-    // chunks, so a droplet's chunks stay the same alone and in a batch.
+    // K_corr (no reductions either) and K_div take chunks of about this depth
+    // in the same batches.  K_div's sums, which set the k = 0 coefficient and
+    // sigma, stay per tuned chunk (Drop::part), so they do not change either.
     static constexpr int kBatchCorrTz = 32;
     bool batch_rhs_layout() const
     {
@@ -505,12 +504,16 @@ struct Engine {
                     for (int bx = 0; bx < (N + tx - 1) / tx; ++bx)
                         w[T_RHS].push_back(Work{si, bx, by, zb | (ze << 16)});
             }
+            // divergence: the droplet's tuned z-chunks are its reduction blocks
+            // (Drop::part); an item covers g consecutive blocks of one tile
+            // (one, except in a device-filling batch)
             const int ntile = ((N + kProjTX - 1) / kProjTX) * ((N + kProjTY - 1) / kProjTY);
             const int nzp = (N + s.proj_tz - 1) / s.proj_tz;
-            int ord = 0;
-            for (int k = 0; k < nzp; ++k) {
-                const int zb = k * N / nzp, ze = (k + 1) * N / nzp;
-                for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), ord++});
+            const int g = batch && s.prm.corr_tz <= 0 ? std::max(1, std::min(nzp, kBatchCorrTz * nzp / N)) : 1;
+            for (int k0 = 0; k0 < nzp; k0 += g) {
+                const int k1 = std::min(nzp, k0 + g);
+                const int zb = k0 * N / nzp, ze = k1 * N / nzp;
+                for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), k0 * ntile + t});
             }
             const int ctz = batch && s.prm.corr_tz <= 0 ? std::min(std::max(s.proj_tz, kBatchCorrTz), N) : s.proj_tz;
             const int nzc = (N + ctz - 1) / ctz;
@@ -518,8 +521,9 @@ struct Engine {
                 const int zb = k * N / nzc, ze = (k + 1) * N / nzc;
                 for (int t = 0; t < ntile; ++t) w[T_CORR].push_back(Work{si, t, zb | (ze << 16), 0});
             }
-            // k_proj_stats sums the divergence items' FP64 partial sums
-            h_drops[si].npart = ord;
+            // k_proj_stats sums the blocks' FP64 partial sums in block order
+            h_drops[si].npart = nzp * ntile;
+            h_drops[si].nzp = nzp;
             w[T_STAT].push_back(Work{si, 0, 0, 0});
             const int sidx = size_index(N);
             const int L = fft_lines_per_group(N);

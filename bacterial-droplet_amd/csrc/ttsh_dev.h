@@ -62,10 +62,14 @@ struct Drop {
     float* vs[2][3];
     float* ps[2][3];
     float2* c;
-    // FP64 sums per divergence work item, kPartSums each: s.x, s.y, s.x^2,
-    // s.y^2 over the item's cells and |v*|^2, |P*|^2 over the same cells
+    // FP64 sums per reduction block, kPartSums each: s.x, s.y, s.x^2, s.y^2
+    // over the block's cells and |v*|^2, |P*|^2 over the same cells.  A block
+    // is one divergence tile over one of nzp z-ranges (the droplet's tuned
+    // z-chunks); npart = tiles * nzp, ordered by z-range, then tile.  A
+    // divergence work item covers one or more whole blocks of one tile.
     double* part;
     int npart;
+    int nzp;
     // Per step, from the sums of all items (k_proj_stats): stat[0..1] the
     // k = 0 coefficient of s/sigma (FP64); sigma the power-of-two scale of
     // the imaginary part.  The packed transform carries div v* and div P*/
