@@ -64,8 +64,8 @@ typedef struct ttsh_params {
                               0 = tuned default.  With the defaults, a batch
                               whose droplets fill the device runs K_rhs with
                               the 32x8 tile and ranges of at least 32, and the
-                              corrector with ranges of at least 32; results do
-                              not change. */
+                              divergence and corrector with ranges of about 32;
+                              results do not change. */
 } ttsh_params;
 
 typedef struct ttsh_sim ttsh_sim;
