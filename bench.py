@@ -102,8 +102,9 @@ def cpu_baseline(p, phi, P, v, min_s=10.0, max_steps=10000):
             "sample": f"FP64 oracle (OpenMP), the same {p.Nx}^3 droplet, {n} steps after 1 warm-up step, {dt:.1f} s"}
 
 
-def batch_sweep(ttsh, steps, seed=1004, ndrop=64):
-    """Batched 64-droplet sweep vs the plain step one droplet at a time."""
+def batch_sweep(ttsh, steps, seed=1004, ndrop=64, plain=True):
+    """Batched 64-droplet sweep vs the plain step one droplet at a time
+    (plain=False: the batched part only)."""
     rng = np.random.default_rng(seed)
     Ls = rng.choice([48, 64, 96, 128], size=ndrop)
     base = ttsh.default_params()
@@ -126,6 +127,11 @@ def batch_sweep(ttsh, steps, seed=1004, ndrop=64):
     b.synchronize()
     tb = time.perf_counter() - t0
     b.close()
+    full = 2000
+    sph_b = ndrop / (tb * full / steps) * 3600.0
+    if not plain:
+        return {"droplets": ndrop, "cells": cells, "steps_timed": steps, "batched_sims_per_hour": round(sph_b, 2),
+                "batched_mcell_steps_per_s": round(cells * steps / tb / 1e6, 1)}
     # plain step, one droplet at a time (same droplets, same steps)
     tp = 0.0
     for p, L, s in cases:
@@ -138,8 +144,6 @@ def batch_sweep(ttsh, steps, seed=1004, ndrop=64):
             sim.step(steps)
             sim.synchronize()
             tp += time.perf_counter() - t0
-    full = 2000
-    sph_b = ndrop / (tb * full / steps) * 3600.0
     sph_p = ndrop / (tp * full / steps) * 3600.0
     return {"droplets": ndrop, "cells": cells, "steps_timed": steps,
             "batched_sims_per_hour": round(sph_b, 2), "plain_sims_per_hour": round(sph_p, 2),
@@ -157,6 +161,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--batch-steps", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched part of the sweep (for kernel traces of the batch) and print it")
     ap.add_argument("--grid", type=int, default=128, choices=sorted(WORKLOADS),
                     help="single-droplet workload: 128 (default) or 256")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -180,6 +186,9 @@ def main():
         del src, dst
 
     import ttsh3d as ttsh
+    if args.batch_only:
+        print(json.dumps({"batch_sweep": batch_sweep(ttsh, args.batch_steps, plain=False)}), flush=True)
+        return
     p, phi, P, v = workload(ttsh, args.grid)
     N = p.Nx
     cells = N ** 3
