@@ -617,6 +617,10 @@ struct Engine {
         // fixed grids: enough work-groups to fill the device, never more than
         // a table can hold
         for (int t = 0; t < kNTables; ++t) ggrid[t] = std::max(0, std::min(tab[t].cap, cus * 8));
+        // the stencil kernels' items differ in cost (z-depths, partial tiles):
+        // one work-group per table slot leaves their balance to the hardware
+        // dispatcher instead of a fixed stride over the items
+        for (int t : {T_RHS, T_PROJ, T_CORR}) ggrid[t] = ((tab[t].cap + kXcds - 1) / kXcds) * kXcds;
         gsizes = sizes_present();
         gxy_mask = xy_mask;
         hipGraph_t graph = nullptr;
