@@ -120,6 +120,10 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
     // per position: phi, Px, Py, Pz x (f, dz, sz) | vx, vy, vz x (f, dz, sz) |
     // Sigma(9, [3a+b]), mu(3), mu0 of one plane
     constexpr int SP = 13, SV = 9, SS = 13;
+    // One-group tiles (MINB 1) are held to two waves per SIMD by LDS, so they
+    // can spend registers on unrolling the z-march by its 3-slot rotation,
+    // which removes the register moves of the rotation.
+    constexpr bool kUnroll3 = MINB == 1;
     __shared__ float zp[A2 * SP];
     __shared__ float zv[A1 * SV];
     __shared__ float sM[A1 * SS];
@@ -283,17 +287,22 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                 asm volatile("" : "+v"(i));
                 if (i >= 0) zprep(zv + i, vinr[m]);
             }
-            // roll the planes and prefetch plane zi+1 into the newest slot
-            auto roll = [](float* t) { t[0] = t[1]; t[1] = t[2]; };
+            if constexpr (kUnroll3) {
+                // prefetch plane zi+1 into the slot of plane zi-2
+                if (zi <= ze) load_plane(S0, zi + 1, o2, q2, o1, q1);
+            } else {
+                // roll the planes and prefetch plane zi+1 into the newest slot
+                auto roll = [](float* t) { t[0] = t[1]; t[1] = t[2]; };
 #pragma unroll
-            for (int f = 0; f < 4; ++f) roll(in[f]);
+                for (int f = 0; f < 4; ++f) roll(in[f]);
 #pragma unroll
-            for (int m = 0; m < NR2; ++m) roll(inr[m]);
+                for (int m = 0; m < NR2; ++m) roll(inr[m]);
 #pragma unroll
-            for (int f = 0; f < 3; ++f) roll(vin[f]);
+                for (int f = 0; f < 3; ++f) roll(vin[f]);
 #pragma unroll
-            for (int m = 0; m < NR1; ++m) roll(vinr[m]);
-            if (zi <= ze) load_plane(2, zi + 1, o2, q2, o1, q1);
+                for (int m = 0; m < NR1; ++m) roll(vinr[m]);
+                if (zi <= ze) load_plane(2, zi + 1, o2, q2, o1, q1);
+            }
             __syncthreads();
 
             // ---- 2. intermediates of plane zm over the 1-cell apron --------
@@ -448,25 +457,46 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                     stb(ops[b] + pz, go, cur[3 + b] + co.Dt * (co.eta * aL[S0][b]));
                 }
             }
-            // rotate the output planes: zm becomes the next zm-1
-            aL0[0] = aL0[1];
-            aL0[1] = aL0[2];
-            aL0[2] = 0.f;
+            if constexpr (kUnroll3) {
+                // slot S0 becomes plane zm+2 in the next phase
+                aL0[S0] = 0.f;
 #pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                aS[0][b] = aS[1][b];
-                aS[1][b] = aS[2][b];
-                aS[2][b] = 0.f;
-                aL[0][b] = aL[1][b];
-                aL[1][b] = aL[2][b];
-                aL[2][b] = 0.f;
+                for (int b = 0; b < 3; ++b) {
+                    aS[S0][b] = 0.f;
+                    aL[S0][b] = 0.f;
+                }
+            } else {
+                // rotate the output planes: zm becomes the next zm-1
+                aL0[0] = aL0[1];
+                aL0[1] = aL0[2];
+                aL0[2] = 0.f;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    aS[0][b] = aS[1][b];
+                    aS[1][b] = aS[2][b];
+                    aS[2][b] = 0.f;
+                    aL[0][b] = aL[1][b];
+                    aL[1][b] = aL[2][b];
+                    aL[2][b] = 0.f;
+                }
             }
 #pragma unroll
             for (int j = 0; j < 7; ++j) cur[j] = nxt[j];
         };
 
         // z-march over zi = zb .. ze+1
-        for (int zi = zb; zi <= ze + 1; ++zi) body(Ph<0>{}, zi);
+        if constexpr (kUnroll3) {
+            for (int zi = zb;;) {
+                body(Ph<0>{}, zi);
+                if (++zi > ze + 1) break;
+                body(Ph<1>{}, zi);
+                if (++zi > ze + 1) break;
+                body(Ph<2>{}, zi);
+                if (++zi > ze + 1) break;
+            }
+        } else {
+            for (int zi = zb; zi <= ze + 1; ++zi) body(Ph<0>{}, zi);
+        }
     }
 }
 
