@@ -366,9 +366,12 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
             // or three (those of P_b, with mu0 in part 0).  Otherwise each of
             // the first NRING threads computes all 13 of one position.  All use
             // the own cell's expressions.
-            constexpr int NPART = 4 * NRING <= NT ? 4 : 3 * NRING <= NT ? 3 : 1;
-            if (ti < NPART * NRING) {
-                const int f = ti / NRING, r = ti % NRING;
+            // two waves with a ring of at most 64 positions: one part per wave
+            constexpr bool kWaveParts = !(3 * NRING <= NT) && NT == 128 && NRING <= 64;
+            constexpr int NPART = 4 * NRING <= NT ? 4 : 3 * NRING <= NT ? 3 : kWaveParts ? 2 : 1;
+            constexpr int PSTR = kWaveParts ? 64 : NRING;  // lane stride between parts
+            if (ti % PSTR < NRING && ti < NPART * PSTR) {
+                const int f = ti / PSTR, r = ti % PSTR;
                 int lx1, ly1;
                 if (r < W1) { lx1 = r; ly1 = 0; }
                 else if (r < 2 * W1) { lx1 = r - W1; ly1 = H1 - 1; }
@@ -397,6 +400,15 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
                 } else if constexpr (NPART == 3) {
                     if (f == 0) mu0();
                     pint(f, (1 + f) * NZ);
+                } else if constexpr (NPART == 2) {
+                    if (f == 0) {  // wave 0: mu0 and P_x
+                        mu0();
+                        pint(0, 1 * NZ);
+                    } else {  // wave 1: P_y and P_z
+                        pint(1, 2 * NZ);
+                        __builtin_amdgcn_sched_barrier(0);
+                        pint(2, 3 * NZ);
+                    }
                 } else {
                     mu0();
 #pragma unroll
