@@ -113,7 +113,7 @@ def test_device_filling_batch_layout_equals_alone(ttsh):
     layout (32x8 tiles, z-chunks of 32), more than two rounds of three groups
     on 256 CUs, so the engine runs K_rhs that way and K_corr with chunks of 32.
     Alone, each droplet runs its tuned 32x16 tile with chunks of 8 or 16
-    planes and K_corr with 8 or 11.  Neither kernel reduces anything, so the
+    planes and K_corr with 8 or 16.  Neither kernel reduces anything, so the
     results are bitwise equal."""
     log("[device_filling_batch_layout_equals_alone]")
     steps = 6
