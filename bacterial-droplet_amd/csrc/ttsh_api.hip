@@ -446,10 +446,10 @@ struct Engine {
 
     int64_t remaining(const Slot& s) const { return s.limit < 0 ? INT64_MAX : std::max<int64_t>(0, s.limit - s.steps); }
 
-    // K_rhs layout for a batch.  The tuned tile and z-depth of a size are
-    // chosen for that droplet alone, with enough z-chunks to fill the device;
-    // each chunk re-reads two warm-up planes.  A batch that fills the device
-    // by itself runs K_rhs with the 32x8 tile (three groups per CU) and
+    // Layout of a batch that fills the device.  The tuned tile and z-depth of
+    // a size are chosen for that droplet alone, with enough z-chunks to fill
+    // the device; each chunk re-reads two warm-up planes.  A batch that fills
+    // the device by itself runs K_rhs with the 32x8 tile (three groups per CU) and
     // chunks of at least kBatchRhsTz planes, which cuts the warm-up share and
     // gives items of nearly equal cost.  It applies when at least two
     // droplets are active, none asks for a tile, every one fits 32x8, and the
@@ -460,7 +460,7 @@ struct Engine {
     // in the same batches.  K_div's sums, which set the k = 0 coefficient and
     // sigma, stay per tuned chunk (Drop::part), so they do not change either.
     static constexpr int kBatchCorrTz = 32;
-    bool batch_rhs_layout() const
+    bool batch_layout() const
     {
         int active = 0;
         int64_t items = 0;
@@ -478,7 +478,7 @@ struct Engine {
     int rebuild(int steps_this_call)
     {
         std::vector<Work> w[kNTables];
-        const bool batch = batch_rhs_layout();
+        const bool batch = batch_layout();
         const int rv = batch ? (int)RHS_32x8 : (rhs_variant < 0 ? 0 : rhs_variant);
         if (rv != rhs_active) {
             HIPCHK(hipStreamSynchronize(st));
