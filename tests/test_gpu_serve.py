@@ -132,3 +132,29 @@ def test_device_filling_batch_layout_equals_alone(ttsh):
         ref = alone(ttsh, N, cA, La, seed, steps)
         for a, c in zip(ref, g):
             assert np.array_equal(a, c), f"{N}^3 droplet seed {seed} differs between batch and alone"
+
+
+def test_layout_switch_mid_run_equals_alone(ttsh):
+    """Six 128^3 and two 96^3 droplets start in the device-filling layout
+    (1,752 K_rhs items at depth 32).  Three 128^3 droplets stop after 8 steps,
+    leaving 984 items, so the engine returns to the tuned layout (and
+    re-captures its graphs) while the others run on to 40 steps.  Every
+    droplet still equals its run alone, bitwise."""
+    log("[layout_switch_mid_run_equals_alone]")
+    cases = [(128, -1.0 - 0.2 * i, 1.0 + 0.05 * i, 720 + i, 8 if i < 3 else 40) for i in range(6)]
+    cases += [(96, -1.2, 1.1, 730, 40), (96, -0.8, 0.9, 731, 40)]
+    f = lambda a: a.astype(np.float32)
+    with ttsh.Batch(max_slots=len(cases), use_graph=True) as b:
+        slots = []
+        for N, cA, La, seed, lim in cases:
+            p = ttsh.default_params(N=N, dt0=0.01, cA=cA, La=La)
+            phi, P, v = ttsh.init_droplet(N, 0.3 * N, 2.0, 0, seed)
+            slots.append(b.admit(p, f(phi), f(P), f(v), dtype=ttsh.F32, limit=lim))
+        b.step(40)
+        for s, case in zip(slots, cases):
+            assert b.steps_done(s) == case[4]
+        got = [b.get_state(s) for s in slots]
+    for (N, cA, La, seed, lim), g in zip(cases, got):
+        ref = alone(ttsh, N, cA, La, seed, lim)
+        for a, c in zip(ref, g):
+            assert np.array_equal(a, c), f"{N}^3 droplet seed {seed} ({lim} steps) differs between batch and alone"
