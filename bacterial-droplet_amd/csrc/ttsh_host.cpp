@@ -435,24 +435,34 @@ int write_conf_host(const char* path, int N, int expo_data, const float* phi, co
     FILE* f = std::fopen(path, "w");
     if (!f) return fail(TTSH_EIO, std::string("cannot write ") + path);
     // ExpoConf loop order (TTSH3D.cu:975-981): i < Ny (y), j < Nx (x), k < Nz
-    // (z).  Rows i are formatted by a few threads at a time and written in
-    // order.
+    // (z).  Rows i are formatted a group of nt at a time, by nt threads, into
+    // one of two buffer sets; while this thread writes a group, the next one
+    // is being formatted into the other set.
     const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-    std::vector<std::string> rows(nt);
-    std::vector<size_t> len(nt, 0);
-    for (int i0 = 0; i0 < N; i0 += nt) {
+    std::vector<std::string> rows[2] = {std::vector<std::string>(nt), std::vector<std::string>(nt)};
+    std::vector<size_t> len[2] = {std::vector<size_t>(nt, 0), std::vector<size_t>(nt, 0)};
+    auto format_group = [&](int b, int i0) {
         const int cnt = std::min(nt, N - i0);
         std::vector<std::thread> th;
         for (int t = 1; t < cnt; ++t)
-            th.emplace_back([&, t] { len[t] = format_row(rows[t], N, expo_data, i0 + t, phi, P, v); });
-        len[0] = format_row(rows[0], N, expo_data, i0, phi, P, v);
+            th.emplace_back([&, b, t, i0] { len[b][t] = format_row(rows[b][t], N, expo_data, i0 + t, phi, P, v); });
+        len[b][0] = format_row(rows[b][0], N, expo_data, i0, phi, P, v);
         for (auto& x : th) x.join();
-        for (int t = 0; t < cnt; ++t) {
-            if (len[t] && std::fwrite(rows[t].data(), 1, len[t], f) != len[t]) {
-                std::fclose(f);
-                return fail(TTSH_EIO, std::string("write failed: ") + path);
-            }
-        }
+    };
+    format_group(0, 0);
+    bool ok = true;
+    for (int i0 = 0, b = 0; i0 < N; i0 += nt, b ^= 1) {
+        std::thread next;
+        if (i0 + nt < N) next = std::thread(format_group, b ^ 1, i0 + nt);
+        const int cnt = std::min(nt, N - i0);
+        for (int t = 0; t < cnt && ok; ++t)
+            if (len[b][t] && std::fwrite(rows[b][t].data(), 1, len[b][t], f) != len[b][t]) ok = false;
+        if (next.joinable()) next.join();
+        if (!ok) break;
+    }
+    if (!ok) {
+        std::fclose(f);
+        return fail(TTSH_EIO, std::string("write failed: ") + path);
     }
     if (std::fclose(f) != 0) return fail(TTSH_EIO, std::string("close failed: ") + path);
     return TTSH_OK;
