@@ -483,41 +483,162 @@ int write_conf_bin_host(const char* path, int N, int m, double T, int64_t steps,
 }
 
 // Level-contour radius R(theta) (characteristicVelocity.py:75-122 on the
-// centroid slice).  The reference takes the vertices of matplotlib's contour
-// paths; here the vertex set is every level crossing of a horizontal or
-// vertical grid edge, interpolated linearly, in row-major edge order.
-int contour_radius(int N, const float* field, double level, double* R, double* center)
-{
-    std::vector<double> px, py;
-    auto F = [&](int r, int c) { return (double)field[(size_t)r * N + c] - level; };
-    auto cross = [](double a, double b) { return (a < 0.0) != (b < 0.0); };
-    for (int r = 0; r < N; ++r) {
-        for (int c = 0; c < N; ++c) {
-            const double a = F(r, c);
-            if (c + 1 < N) {
-                const double b = F(r, c + 1);
-                if (cross(a, b)) {
-                    px.push_back(c + a / (a - b));
-                    py.push_back(r);
-                }
-            }
-            if (r + 1 < N) {
-                const double b = F(r + 1, c);
-                if (cross(a, b)) {
-                    px.push_back(c);
-                    py.push_back(r + a / (a - b));
-                }
-            }
+// centroid slice).  The reference calls plt.contour(phi, levels=[0.5]) and
+// takes the vertices of every contour path, in path order, with the centre
+// at the mean of the vertices of the LAST path (:79-87).  The paths here are
+// traced by marching squares with the conventions of matplotlib's default
+// contour generator (contourpy's "mpl2014"), so the vertex list is the same:
+//   - a point is above the level when its value is > level; an edge is
+//     crossed when its ends differ, at x0 + (level - z0)/(z1 - z0);
+//   - lines run with the values above the level on their left;
+//   - lines that enter through the array border come first, found by
+//     scanning quads row by row and, in a quad, its bottom, left, right and
+//     top edges; then closed loops, found by scanning points row by row and,
+//     at a point, the edge up before the edge to the right;
+//   - a closed loop repeats its first vertex at its end;
+//   - a saddle quad (corners alternating) joins its above-level corners when
+//     the mean of its four corners is above the level.
+// The theta scan then takes, for theta = -180, -177, ..., 177 degrees, the
+// radius of the first vertex whose polar angle about the centre is nearest.
+namespace {
+
+struct ContourTracer {
+    int nx, ny;
+    const float* z;
+    double level;
+    std::vector<unsigned char> hi, vis_h, vis_v;  // point above level; edge visited
+    std::vector<double> px, py;                    // vertices of all lines, in order
+    size_t last_begin = 0;                         // first vertex of the last line
+
+    double Z(int x, int y) const { return (double)z[(size_t)y * nx + x]; }
+    bool H(int x, int y) const { return hi[(size_t)y * nx + x] != 0; }
+    // edge: kind 0 = horizontal (x,y)-(x+1,y), 1 = vertical (x,y)-(x,y+1)
+    bool crossed(int k, int x, int y) const { return k == 0 ? H(x, y) != H(x + 1, y) : H(x, y) != H(x, y + 1); }
+    unsigned char& visited(int k, int x, int y) { return (k == 0 ? vis_h : vis_v)[(size_t)y * nx + x]; }
+    void emit(int k, int x, int y)
+    {
+        if (k == 0) {
+            const double z0 = Z(x, y), z1 = Z(x + 1, y);
+            px.push_back(x + (level - z0) / (z1 - z0));
+            py.push_back(y);
+        } else {
+            const double z0 = Z(x, y), z1 = Z(x, y + 1);
+            px.push_back(x);
+            py.push_back(y + (level - z0) / (z1 - z0));
         }
     }
+    // the quad a line crossing edge e enters, with the above-level side on its left
+    void forward(int k, int x, int y, int& qx, int& qy) const
+    {
+        if (k == 0) {
+            qx = x;
+            qy = H(x, y) ? y : y - 1;
+        } else {
+            qx = H(x, y + 1) ? x : x - 1;
+            qy = y;
+        }
+    }
+    bool inside(int qx, int qy) const { return qx >= 0 && qy >= 0 && qx < nx - 1 && qy < ny - 1; }
+    // quad edges by side: 0 bottom, 1 top, 2 left, 3 right
+    static void side(int qx, int qy, int s, int& k, int& x, int& y)
+    {
+        switch (s) {
+        case 0: k = 0; x = qx; y = qy; break;
+        case 1: k = 0; x = qx; y = qy + 1; break;
+        case 2: k = 1; x = qx; y = qy; break;
+        default: k = 1; x = qx + 1; y = qy; break;
+        }
+    }
+    // exit side of quad q for a line entering through side s_in
+    int exit_side(int qx, int qy, int s_in) const
+    {
+        int cr[4], n = 0;
+        for (int s = 0; s < 4; ++s) {
+            if (s == s_in) continue;
+            int k, x, y;
+            side(qx, qy, s, k, x, y);
+            if (crossed(k, x, y)) cr[n++] = s;
+        }
+        if (n == 1) return cr[0];
+        // saddle: turn to the side on the left or the right of the entry
+        static const int left_of[4] = {2, 3, 1, 0}, right_of[4] = {3, 2, 0, 1};
+        const double zmid = 0.25 * (Z(qx, qy) + Z(qx + 1, qy) + Z(qx, qy + 1) + Z(qx + 1, qy + 1));
+        return zmid > level ? right_of[s_in] : left_of[s_in];
+    }
+    static int side_of(int qx, int qy, int k, int x, int y)
+    {
+        if (k == 0) return y == qy ? 0 : 1;
+        return x == qx ? 2 : 3;
+    }
+    void follow(int k0, int x0, int y0)
+    {
+        last_begin = px.size();
+        emit(k0, x0, y0);
+        visited(k0, x0, y0) = 1;
+        int k = k0, x = x0, y = y0, qx, qy;
+        forward(k, x, y, qx, qy);
+        while (inside(qx, qy)) {
+            const int s = exit_side(qx, qy, side_of(qx, qy, k, x, y));
+            side(qx, qy, s, k, x, y);
+            if (k == k0 && x == x0 && y == y0) {  // back at the start: close the loop
+                px.push_back(px[last_begin]);
+                py.push_back(py[last_begin]);
+                return;
+            }
+            visited(k, x, y) = 1;
+            emit(k, x, y);
+            forward(k, x, y, qx, qy);
+        }
+    }
+    void run()
+    {
+        const size_t n = (size_t)nx * ny;
+        hi.assign(n, 0);
+        vis_h.assign(n, 0);
+        vis_v.assign(n, 0);
+        for (size_t q = 0; q < n; ++q) hi[q] = (double)z[q] > level;
+        // lines entering through the border, by quad, sides bottom, left, right, top
+        static const int order[4] = {0, 2, 3, 1};
+        for (int qy = 0; qy < ny - 1; ++qy)
+            for (int qx = 0; qx < nx - 1; ++qx)
+                for (int o = 0; o < 4; ++o) {
+                    int k, x, y;
+                    side(qx, qy, order[o], k, x, y);
+                    const bool border = k == 0 ? (y == 0 || y == ny - 1) : (x == 0 || x == nx - 1);
+                    if (!border || visited(k, x, y) || !crossed(k, x, y)) continue;
+                    int fx, fy;
+                    forward(k, x, y, fx, fy);
+                    if (fx == qx && fy == qy) follow(k, x, y);
+                }
+        // closed loops, by point, the edge up before the edge to the right
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                if (y + 1 < ny && !visited(1, x, y) && crossed(1, x, y)) follow(1, x, y);
+                if (x + 1 < nx && !visited(0, x, y) && crossed(0, x, y)) follow(0, x, y);
+            }
+    }
+};
+
+}  // namespace
+
+int contour_radius(int N, const float* field, double level, double* R, double* center)
+{
+    ContourTracer t;
+    t.nx = t.ny = N;
+    t.z = field;
+    t.level = level;
+    t.run();
+    const std::vector<double>& px = t.px;
+    const std::vector<double>& py = t.py;
     if (px.empty()) return fail(TTSH_EINVAL, "the level is not crossed");
     double cx = 0.0, cy = 0.0;
-    for (size_t q = 0; q < px.size(); ++q) {
+    for (size_t q = t.last_begin; q < px.size(); ++q) {
         cx += px[q];
         cy += py[q];
     }
-    cx /= (double)px.size();
-    cy /= (double)px.size();
+    const double cnt = (double)(px.size() - t.last_begin);
+    cx /= cnt;
+    cy /= cnt;
     std::vector<double> pr(px.size()), pt(px.size());
     for (size_t q = 0; q < px.size(); ++q) {
         const double dx = px[q] - cx, dy = py[q] - cy;

@@ -183,13 +183,16 @@ int ttsh_get_slice(ttsh_sim* s, int z, float* out, int* z_used);
 
 /* New, host only: radius R(theta) of the level contour of an N x N field,
  * as characteristicVelocity.py:75-122 extracts it from the centroid slice:
- * the level crossings on every horizontal and vertical grid edge (linear
- * interpolation, no periodic wrap), their mean as the centre, and for
- * theta = -180, -177, ..., 177 degrees the distance of the crossing whose
- * polar angle about the centre is nearest (the first in row-major edge
- * order on ties).  field is [row][col] with x = col, y = row; R receives 120
- * values, center (may be NULL) the centre (x, y).  TTSH_EINVAL if the level
- * is never crossed. */
+ * the vertices of every path of plt.contour(field, levels=[level]) in path
+ * order (traced here with the conventions of matplotlib's default contour
+ * generator, mpl2014: linear interpolation on grid edges, no periodic wrap,
+ * lines through the border first, closed loops repeating their first
+ * vertex), the centre at the mean of the vertices of the last path (:85),
+ * and for theta = -180, -177, ..., 177 degrees the distance of the vertex
+ * whose polar angle about the centre is nearest (the first on ties).
+ * field is [row][col] with x = col, y = row; R receives 120 values, center
+ * (may be NULL) the centre (x, y).  TTSH_EINVAL if the level is never
+ * crossed. */
 int ttsh_contour_radius(int N, const float* field, double level, double* R, double* center);
 
 /* Output options of ttsh_run_ex (bitwise or). */

@@ -49,36 +49,35 @@ def slice_fields(phi, P, v, z, h):
     return out
 
 
+def contour_paths(field, level=0.5):
+    """The vertex arrays of plt.contour(field, levels=[level]) in path order,
+    as characteristicVelocity.py:77-79 obtains them.  The reference reads
+    contour_set.collections[0].get_paths(), an API matplotlib 3.10 removed;
+    cs.allsegs[0] holds the same vertex arrays.  Needs matplotlib (default
+    contour algorithm)."""
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    cs = plt.contour(np.asarray(field, np.float64), levels=[level], colors="black")
+    plt.close()
+    return [np.asarray(s, np.float64) for s in cs.allsegs[0] if len(s)]
+
+
 def contour_radius(field, level=0.5):
-    """R(theta) of extract_contour_polar (characteristicVelocity.py:75-122)
-    with the vertex set taken as every level crossing of a horizontal or
-    vertical grid edge (linear interpolation, x = column, y = row, edges
-    visited row-major, at each point first the edge to the right, then the
-    edge below), the centre as their mean, and for theta = -180, -177, ...,
-    177 degrees the radius of the first crossing with the nearest polar
-    angle."""
-    f = np.asarray(field, np.float64) - level
-    N = f.shape[0]
-    px, py = [], []
-    for r in range(N):
-        for c in range(N):
-            a = f[r, c]
-            if c + 1 < N:
-                b = f[r, c + 1]
-                if (a < 0.0) != (b < 0.0):
-                    px.append(c + a / (a - b))
-                    py.append(float(r))
-            if r + 1 < N:
-                b = f[r + 1, c]
-                if (a < 0.0) != (b < 0.0):
-                    px.append(float(c))
-                    py.append(r + a / (a - b))
-    px, py = np.array(px), np.array(py)
-    cx, cy = px.sum() / len(px), py.sum() / len(py)
-    dx, dy = px - cx, py - cy
-    pr = np.sqrt(dx * dx + dy * dy)
+    """extract_contour_polar (characteristicVelocity.py:75-122): all vertices
+    of all paths in order, the centre at the mean of the LAST path's vertices
+    (:85, inside the loop, so the last iteration wins), and for theta = -180,
+    -177, ..., 177 degrees the radius of the first vertex with the nearest
+    polar angle (np.argmin).  Returns (R[120], (cx, cy))."""
+    paths = contour_paths(field, level)
+    if not paths:
+        raise ValueError("the level is not crossed")
+    pts = np.concatenate(paths)
+    cx, cy = np.mean(paths[-1], axis=0)
+    dx, dy = pts[:, 0] - cx, pts[:, 1] - cy
+    pr = np.sqrt(dx ** 2 + dy ** 2)
     pt = np.arctan2(dy, dx)
     R = np.empty(120)
     for a, deg in enumerate(range(-180, 180, 3)):
         R[a] = pr[np.argmin(np.abs(pt - deg / 180 * np.pi))]
-    return R, (cx, cy)
+    return R, (float(cx), float(cy))

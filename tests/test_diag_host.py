@@ -1,5 +1,7 @@
 """Host-side diagnostics: the contour radius of ttsh_contour_radius against
-the restatement in oracle/postproc.py and against shapes with a known
+the reference's algorithm (characteristicVelocity.py:75-122 on matplotlib's
+contour paths: committed fixtures from tests/golden/make_contour_golden.py,
+and matplotlib itself where it imports) and against shapes with a known
 contour.  No GPU needed."""
 import math
 import os
@@ -19,35 +21,78 @@ def droplet_slice(N, R0, xi, cx, cy, ax=1.0, ay=1.0):
     return 0.5 * (1.0 - np.tanh((r - R0) / xi))
 
 
-def test_contour_matches_restatement(ttsh):
-    rng = np.random.default_rng(7)
-    for N, R0, cx, cy in ((64, 20.0, 31.3, 32.7), (48, 11.0, 20.2, 25.9), (128, 40.0, 64.5, 60.1)):
-        f = droplet_slice(N, R0, 2.0, cx, cy) + 0.02 * rng.standard_normal((N, N))
+GOLD = os.path.join(ROOT, "tests", "golden", "contour_golden.npz")
+
+
+def test_contour_matches_golden(ttsh):
+    """R(theta) and the centre against the reference algorithm's output on
+    one droplet, an ellipse, two droplets (centre of the last path), a
+    droplet cut by the border (open lines), an annulus and a random field."""
+    g = np.load(GOLD)
+    for name in g["names"]:
+        f = g[f"field_{name}"]
+        R, c = ttsh.contour_radius(f)
+        assert np.abs(R - g[f"R_{name}"]).max() < 1e-9, name
+        assert np.abs(np.array(c) - g[f"center_{name}"]).max() < 1e-9, name
+
+
+def test_contour_matches_matplotlib_live(ttsh):
+    """The same against matplotlib in this process, on seeded random fields
+    of many sizes (open lines, closed loops, saddles, points on the level)."""
+    pytest.importorskip("matplotlib")
+    rng = np.random.default_rng(11)
+    checked = 0
+    for t in range(120):
+        N = int(rng.integers(3, 40))
+        if t % 3 == 0:
+            f = rng.integers(0, 3, (N, N)) * 0.25 + 0.25  # values on the level
+        else:
+            f = rng.random((N, N))
+            for _ in range(t % 4):
+                f = (f + np.roll(f, 1, 0) + np.roll(f, 1, 1) + np.roll(f, (1, 1), (0, 1))) / 4
         f32 = f.astype(np.float32)
+        try:
+            R_ref, c_ref = postproc.contour_radius(f32.astype(np.float64))
+        except ValueError:
+            with pytest.raises(ttsh.TTSHError):
+                ttsh.contour_radius(f32)
+            continue
         R, c = ttsh.contour_radius(f32)
-        R_ref, c_ref = postproc.contour_radius(f32.astype(np.float64))
-        assert np.allclose(R, R_ref, rtol=0, atol=1e-11)
-        assert abs(c[0] - c_ref[0]) < 1e-11 and abs(c[1] - c_ref[1]) < 1e-11
+        assert np.abs(np.array(c) - np.array(c_ref)).max() < 1e-9, (t, N)
+        # Where two vertices lie at the same polar angle (fields with values
+        # on the level make such ties), which one np.argmin takes depends on
+        # the last bit of arctan2; accept any vertex within 1e-12 of the
+        # nearest angle there.
+        pts = np.concatenate(postproc.contour_paths(f32.astype(np.float64)))
+        dx, dy = pts[:, 0] - c_ref[0], pts[:, 1] - c_ref[1]
+        pr, pt = np.sqrt(dx ** 2 + dy ** 2), np.arctan2(dy, dx)
+        for a, deg in enumerate(range(-180, 180, 3)):
+            d = np.abs(pt - deg / 180 * np.pi)
+            ok = pr[d <= d.min() + 1e-12]
+            assert np.abs(ok - R[a]).min() < 1e-9, (t, N, deg)
+        assert abs(R[np.argmax(np.abs(R - R_ref))] - R_ref[np.argmax(np.abs(R - R_ref))]) < 1e-9 or t % 3 == 0
+        checked += 1
+    assert checked > 60
 
 
 def test_contour_of_a_circle():
     """A tanh droplet of radius R0 centred on a grid point or a cell centre:
-    the crossings are symmetric, so their mean is the droplet's centre and
-    R(theta) ~ R0 at all 120 angles within the interpolation error.  Off such
-    a centre the mean of the edge crossings (like the mean of contour
-    vertices the reference takes) is biased by a fraction of a cell."""
+    R(theta) ~ R0 at all 120 angles.  The centre is the mean of the closed
+    path's vertices, whose first vertex is repeated at its end (as in
+    matplotlib's paths), so it sits (v0 - c)/(n + 1) off the droplet centre
+    c, where v0 is the first vertex and n the number of crossings."""
     import ttsh3d as ttsh
     N, R0 = 96, 30.0
     for cx, cy in ((48.0, 48.0), (47.5, 47.5)):
         f = droplet_slice(N, R0, 2.0, cx, cy).astype(np.float32)
         R, c = ttsh.contour_radius(f)
         assert R.shape == (120,)
-        assert abs(c[0] - cx) < 1e-9 and abs(c[1] - cy) < 1e-9
-        assert np.max(np.abs(R - R0)) < 0.01
+        assert abs(c[0] - cx) < 0.2 and abs(c[1] - cy) < 0.2
+        assert np.max(np.abs(R - R0)) < 0.2
     f = droplet_slice(N, R0, 2.0, 47.25, 48.6).astype(np.float32)
     R, c = ttsh.contour_radius(f)
-    assert abs(c[0] - 47.25) < 0.2 and abs(c[1] - 48.6) < 0.2
-    assert np.max(np.abs(R - R0)) < 0.2
+    assert abs(c[0] - 47.25) < 0.3 and abs(c[1] - 48.6) < 0.3
+    assert np.max(np.abs(R - R0)) < 0.3
 
 
 def test_contour_of_an_ellipse_angles():
