@@ -492,7 +492,9 @@ struct Engine {
             h_drops[si] = s.drop;
             if (!s.used) continue;
             const int64_t rem = remaining(s);
-            h_drops[si].nsteps = (int)std::min<int64_t>(rem, (int64_t)steps_this_call);
+            // a droplet without a step limit never stops inside a launch
+            // sequence, so its descriptor serves any chunk length
+            h_drops[si].nsteps = s.limit < 0 ? kNoLimit : (int)std::min<int64_t>(rem, (int64_t)steps_this_call);
             if (rem <= 0) continue;
             const int N = s.N;
             // z is split into ceil(N/tz) chunks of near-equal depth
@@ -647,6 +649,22 @@ struct Engine {
         return a;
     }
 
+    static constexpr int kNoLimit = 1 << 30;
+
+    // Limited droplets store min(remaining, chunk) as their step count, so
+    // their descriptors must be rebuilt when the chunk length changes.
+    bool any_limited() const
+    {
+        for (auto& s : slots)
+            if (s.used && s.limit >= 0) return true;
+        return false;
+    }
+
+    bool needs_rebuild(int64_t chunk) const
+    {
+        return tables_dirty || limits_bind(chunk) || (chunk != last_chunk && any_limited());
+    }
+
     bool limits_bind(int64_t chunk) const
     {
         for (auto& s : slots)
@@ -675,7 +693,7 @@ struct Engine {
             const bool g = graphs && use_graph && n >= graph_steps;
             const int64_t chunk = g ? graph_steps : std::min<int64_t>(n, 1 << 20);
             // per-droplet step counts of this chunk live in the descriptors
-            if (tables_dirty || chunk != last_chunk || limits_bind(chunk)) {
+            if (needs_rebuild(chunk)) {
                 int rc = rebuild((int)chunk);
                 if (rc) return rc;
             }
@@ -714,7 +732,7 @@ struct Engine {
         while (n > 0 && rc == TTSH_OK) {
             const int64_t m = std::min(n, kChunk);
             normalise_parity();
-            if (tables_dirty || m != last_chunk || limits_bind(m)) rc = rebuild((int)m);
+            if (needs_rebuild(m)) rc = rebuild((int)m);
             for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
                 rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, false, &ev[E * s]);
             if (rc) break;
