@@ -603,6 +603,21 @@ hipError_t launch_fill_edges(const float* const vs[3], const float* const ps[3],
     return hipGetLastError();
 }
 
+// Fill n 32-bit words with a value (vector stores; grid-stride).
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) glb(p)[i] = value;
+}
+
+hipError_t launch_fill_u32(void* p, uint32_t value, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<uint32_t*>(p), value, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_corr(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,
                        hipStream_t st)
 {

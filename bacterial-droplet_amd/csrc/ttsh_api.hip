@@ -152,7 +152,39 @@ struct Engine {
     int last_chunk = -1;
     bool tables_dirty = true;
 
+    // External arena (ttsh_step_noalloc): every device buffer is carved from
+    // [arena, arena + arena_cap) instead of hipMalloc, nothing is freed, and
+    // the stream is the caller's.
+    bool external = false;
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
+
     ~Engine() { release(); }
+
+    hipError_t dev_alloc(void** p, size_t bytes)
+    {
+        if (!external) return hipMalloc(p, bytes);
+        const size_t b = align256(bytes);
+        if (arena_off + b > arena_cap) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        *p = arena + arena_off;
+        arena_off += b;
+        return hipSuccess;
+    }
+    template <typename T>
+    hipError_t dev_alloc(T** p, size_t bytes)
+    {
+        void* v = nullptr;
+        const hipError_t e = dev_alloc(&v, bytes);
+        *p = static_cast<T*>(v);
+        return e;
+    }
+    void dev_free(void* p)
+    {
+        if (!external && p) (void)hipFree(p);
+    }
 
     void drop_graphs()
     {
@@ -164,25 +196,25 @@ struct Engine {
 
     void release()
     {
-        if (st) (void)hipSetDevice(device);
-        if (st) (void)hipStreamSynchronize(st);
+        if (st || external) (void)hipSetDevice(device);
+        if (st || external) (void)hipStreamSynchronize(st);
         drop_graphs();
         for (auto& s : slots)
-            if (s.mem) { (void)hipFree(s.mem); s.mem = nullptr; }
-        for (auto& kv : tw) (void)hipFree(kv.second);
+            if (s.mem) { dev_free(s.mem); s.mem = nullptr; }
+        for (auto& kv : tw) dev_free(kv.second);
         tw.clear();
-        if (d_drops) (void)hipFree(d_drops);
+        dev_free(d_drops);
         for (auto& t : tab)
-            if (t.d) { (void)hipFree(t.d); t.d = nullptr; t.cap = 0; }
-        if (d_counts) (void)hipFree(d_counts);
-        if (d_redpart) (void)hipFree(d_redpart);
-        if (d_redout) (void)hipFree(d_redout);
-        if (d_cpart) (void)hipFree(d_cpart);
-        if (d_csum) (void)hipFree(d_csum);
-        if (d_slice) (void)hipFree(d_slice);
+            if (t.d) { dev_free(t.d); t.d = nullptr; t.cap = 0; }
+        dev_free(d_counts);
+        dev_free(d_redpart);
+        dev_free(d_redout);
+        dev_free(d_cpart);
+        dev_free(d_csum);
+        dev_free(d_slice);
         d_drops = nullptr; d_counts = nullptr; d_redpart = nullptr; d_redout = nullptr;
         d_cpart = nullptr; d_csum = nullptr; d_slice = nullptr; slice_N = 0;
-        if (st) (void)hipStreamDestroy(st);
+        if (st && !external) (void)hipStreamDestroy(st);
         st = nullptr;
     }
 
@@ -199,16 +231,28 @@ struct Engine {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        if (!external) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         slots.resize(nslots);
-        HIPCHK(hipMalloc(&d_drops, sizeof(Drop) * nslots));
+        HIPCHK(dev_alloc(&d_drops, sizeof(Drop) * nslots));
         h_drops.resize(nslots);
-        HIPCHK(hipMalloc(&d_counts, sizeof(int) * kNTables));
-        HIPCHK(hipMalloc(&d_redpart, sizeof(double) * 3 * kRedBlocks));
-        HIPCHK(hipMalloc(&d_redout, sizeof(double) * 3));
-        HIPCHK(hipMalloc(&d_cpart, sizeof(double) * 4 * kRedBlocks));
-        HIPCHK(hipMalloc(&d_csum, sizeof(double) * 4));
+        HIPCHK(dev_alloc(&d_counts, sizeof(int) * kNTables));
+        HIPCHK(dev_alloc(&d_redpart, sizeof(double) * 3 * kRedBlocks));
+        HIPCHK(dev_alloc(&d_redout, sizeof(double) * 3));
+        HIPCHK(dev_alloc(&d_cpart, sizeof(double) * 4 * kRedBlocks));
+        HIPCHK(dev_alloc(&d_csum, sizeof(double) * 4));
         return TTSH_OK;
+    }
+
+    // An engine whose device memory is the arena [base, base + cap) and whose
+    // stream is the caller's (ttsh_step_noalloc).
+    int init_external(int dev, hipStream_t stream, void* base, size_t cap)
+    {
+        external = true;
+        arena = static_cast<char*>(base);
+        arena_cap = cap;
+        arena_off = 0;
+        st = stream;
+        return init(dev, 1, 0, false);
     }
 
     // Diagnostic launches on the engine stream for slot si: the centroid sums
@@ -258,9 +302,9 @@ struct Engine {
         if (z >= s.N) return fail(TTSH_EINVAL, "plane outside the box");
         HIPCHK(hipSetDevice(device));
         if (slice_N != s.N) {
-            if (d_slice) HIPCHK(hipFree(d_slice));
+            dev_free(d_slice);
             d_slice = nullptr;
-            HIPCHK(hipMalloc(&d_slice, sizeof(float) * (9 * (size_t)s.N * s.N + 1)));
+            HIPCHK(dev_alloc(&d_slice, sizeof(float) * (9 * (size_t)s.N * s.N + 1)));
             slice_N = s.N;
         }
         int rc = launch_diag(si, d_csum, z, d_slice);
@@ -285,7 +329,7 @@ struct Engine {
             h[q] = make_float2((float)std::cos(a), (float)-std::sin(a));
         }
         float2* d = nullptr;
-        HIPCHK(hipMalloc(&d, sizeof(float2) * N));
+        HIPCHK(dev_alloc(&d, sizeof(float2) * N));
         HIPCHK(hipMemcpy(d, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
         tw[N] = d;
         *out = d;
@@ -350,7 +394,7 @@ struct Engine {
         }
         HIPCHK(hipSetDevice(device));
         void* mem = nullptr;
-        hipError_t e = hipMalloc(&mem, bytes);
+        hipError_t e = dev_alloc(&mem, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return fail(TTSH_ENOMEM, std::string("hipMalloc of droplet state failed: ") + hipGetErrorString(e));
@@ -399,10 +443,11 @@ struct Engine {
         D.eslot[1] = 1;
         D.faithful = p.faithful_halo ? 1 : 0;
         make_coef(p, D.co);
-        HIPCHK(hipMemsetAsync(mem, 0, bytes, st));
+        // zero state, sigma = 1 until the first step sets it (fill kernels:
+        // no memset node, should a caller capture an external engine)
+        HIPCHK(launch_fill_u32(mem, 0u, bytes / 4, st));
+        HIPCHK(launch_fill_u32(D.sigma, 0x3f800000u, 1, st));
         HIPCHK(hipStreamSynchronize(st));
-        const float one = 1.f;  // sigma until the first step sets it
-        HIPCHK(hipMemcpy(D.sigma, &one, sizeof(float), hipMemcpyHostToDevice));
         s.q = engine_parity;
         s.steps = 0;
         s.limit = -1;
@@ -420,7 +465,7 @@ struct Engine {
         if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipFree(s.mem));
+        dev_free(s.mem);
         used -= s.bytes;
         s = Slot();
         tables_dirty = true;
@@ -542,9 +587,10 @@ struct Engine {
             const int cnt = (int)w[k].size();
             if (cnt > tab[k].cap) {
                 HIPCHK(hipStreamSynchronize(st));
-                if (tab[k].d) HIPCHK(hipFree(tab[k].d));
+                dev_free(tab[k].d);
+                tab[k].d = nullptr;
                 const int cap = std::max(cnt, 256);
-                HIPCHK(hipMalloc(&tab[k].d, sizeof(Work) * cap));
+                HIPCHK(dev_alloc(&tab[k].d, sizeof(Work) * cap));
                 tab[k].cap = cap;
                 drop_graphs();  // captured launches hold the old table
             }
@@ -794,6 +840,49 @@ struct Engine {
         s.steps = 0;
         s.T = s.prm.T0;
         tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    // State in from device memory (FP32, same layouts as upload_state): phi,
+    // P, v and the previous step's predictors vsp, psp, copied on the
+    // engine's stream.
+    int upload_device(int si, const float* phi, const float* P, const float* v, const float* vsp, const float* psp)
+    {
+        Slot& s = slots[si];
+        HIPCHK(hipSetDevice(device));
+        normalise_parity();
+        const size_t n = (size_t)s.N * s.N * s.N, b = n * sizeof(float);
+        const int q = s.q;
+        HIPCHK(hipMemcpyAsync(s.drop.phi[q], phi, b, hipMemcpyDeviceToDevice, st));
+        for (int a = 0; a < 3; ++a) {
+            HIPCHK(hipMemcpyAsync(s.drop.P[a], P + a * n, b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(s.drop.v[a], v + a * n, b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(s.drop.vs[q ^ 1][a], vsp + a * n, b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(s.drop.ps[q ^ 1][a], psp + a * n, b, hipMemcpyDeviceToDevice, st));
+        }
+        const float* vs[3] = {s.drop.vs[q ^ 1][0], s.drop.vs[q ^ 1][1], s.drop.vs[q ^ 1][2]};
+        const float* ps[3] = {s.drop.ps[q ^ 1][0], s.drop.ps[q ^ 1][1], s.drop.ps[q ^ 1][2]};
+        HIPCHK(launch_fill_edges(vs, ps, s.drop.edge, s.drop.eslot[q ^ 1], s.N, st));
+        s.steps = 0;
+        s.T = s.prm.T0;
+        tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    // State out to device memory: phi, P, v and the last step's predictors.
+    int download_device(int si, float* phi, float* P, float* v, float* vs_out, float* ps_out)
+    {
+        Slot& s = slots[si];
+        HIPCHK(hipSetDevice(device));
+        const size_t n = (size_t)s.N * s.N * s.N, b = n * sizeof(float);
+        const int last = s.q ^ 1;  // parity of the last step's predictors
+        HIPCHK(hipMemcpyAsync(phi, s.drop.phi[s.q], b, hipMemcpyDeviceToDevice, st));
+        for (int a = 0; a < 3; ++a) {
+            HIPCHK(hipMemcpyAsync(P + a * n, s.drop.P[a], b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(v + a * n, s.drop.v[a], b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(vs_out + a * n, s.drop.vs[last][a], b, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(ps_out + a * n, s.drop.ps[last][a], b, hipMemcpyDeviceToDevice, st));
+        }
         return TTSH_OK;
     }
 
@@ -1076,6 +1165,43 @@ struct ttsh_sim {
 struct ttsh_batch {
     Engine eng;
 };
+
+// ---- allocation-free stepping (ttsh_step_noalloc) ---------------------------
+//
+// Callers that cannot allocate device memory (code under HIP graph capture,
+// tight serving loops, harnesses that forbid hipMalloc) step a droplet whose
+// state they own in device memory.  Every buffer the step needs beyond that
+// state -- the phi and predictor ping-pong partners, the complex projection
+// buffer, edge columns, divergence sums, sigma, twiddles, descriptor and work
+// tables -- lives in one file-scope device array per GPU, held until the
+// process ends.  The kernels address a droplet's fields as 32-bit offsets
+// from its lowest buffer, so the whole droplet sits in this one array and the
+// caller's state is copied in and out around the steps.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kNoAllocMaxN = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev`: taken once per
+// device with hipGetSymbolAddress (each GPU holds its own copy of the array
+// at its own address; the array's name in host code is not that address).
+int scratch_address(int dev, void** out)
+{
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(TTSH_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return TTSH_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1377,6 +1503,35 @@ int ttsh_batch_synchronize(ttsh_batch* b)
     HIPCHK(hipSetDevice(b->eng.device));
     HIPCHK(hipStreamSynchronize(b->eng.st));
     return TTSH_OK;
+}
+
+int ttsh_step_noalloc(const ttsh_params* p, float* phi, float* P, float* v, float* vstar_prev, float* pstar_prev,
+                      int64_t n, void* stream)
+{
+    if (!p || !phi || !P || !v || !vstar_prev || !pstar_prev) return fail(TTSH_EINVAL, "null argument");
+    if (n < 0) return fail(TTSH_EINVAL, "negative step count");
+    int rc = ttsh_validate_params(p);
+    if (rc) return rc;
+    if (p->Nx > kNoAllocMaxN)
+        return fail(TTSH_EINVAL, "ttsh_step_noalloc: N=" + std::to_string(p->Nx) + " exceeds its maximum N=" +
+                                     std::to_string(kNoAllocMaxN) + " (the scratch array holds " +
+                                     std::to_string(kScratchBytes >> 20) + " MiB)");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TTSH_EINVAL, "device index out of range");
+    // calls on one device share its scratch array: serialise them
+    std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+    void* base = nullptr;
+    rc = scratch_address(dev, &base);
+    if (rc) return rc;
+    Engine e;
+    rc = e.init_external(dev, static_cast<hipStream_t>(stream), base, kScratchBytes);
+    if (!rc) rc = e.alloc_slot(0, *p);
+    if (!rc) rc = e.upload_device(0, phi, P, v, vstar_prev, pstar_prev);
+    if (!rc) rc = e.step(n, false);
+    if (!rc) rc = e.download_device(0, phi, P, v, vstar_prev, pstar_prev);
+    if (!rc) HIPCHK(hipStreamSynchronize(e.st));
+    return rc;
 }
 
 }  // extern "C"

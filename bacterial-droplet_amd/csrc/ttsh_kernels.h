@@ -39,6 +39,8 @@ hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwo
 // (the layout of Drop::edge).
 hipError_t launch_fill_edges(const float* const vs[3], const float* const ps[3], float* edge, int slot, int N,
                              hipStream_t st);
+// n 32-bit words at p set to value.
+hipError_t launch_fill_u32(void* p, uint32_t value, size_t n, hipStream_t st);
 // v = v* - Dt D p_v, P = P* - Dt D p_p from p_v + i p_p.
 hipError_t launch_corr(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,
                        hipStream_t st);

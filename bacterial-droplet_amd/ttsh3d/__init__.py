@@ -117,6 +117,7 @@ def lib():
         "ttsh_batch_retire": (C.c_int, [vp, C.c_int]),
         "ttsh_batch_usage": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
         "ttsh_batch_synchronize": (C.c_int, [vp]),
+        "ttsh_step_noalloc": (C.c_int, [P, vp, vp, vp, vp, vp, i64, vp]),
         "ttsh_tuning_reload": (C.c_int, [C.c_char_p]),
         "ttsh_tuning_lookup": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "ttsh_last_error": (C.c_char_p, []),
@@ -198,6 +199,17 @@ def init_droplet(N: int, R: float, xi: float, pmode: int = 0, seed: int = 0):
     _check(lib().ttsh_init_droplet(int(N), float(R), float(xi), int(pmode), C.c_uint64(seed),
                                    _ptr(phi), _ptr(P), _ptr(v)))
     return phi, P, v
+
+
+def step_noalloc(params: "Params", phi_ptr: int, P_ptr: int, v_ptr: int, vstar_prev_ptr: int, pstar_prev_ptr: int,
+                 n: int, stream: int = 0) -> None:
+    """ttsh_step_noalloc: n steps of a droplet whose FP32 state lives in
+    caller-owned device memory (raw device addresses; phi N^3, the others
+    3*N^3), with no device allocation by the library.  On return P_ptr..
+    hold the new state and the *_prev buffers the last step's predictors."""
+    _check(lib().ttsh_step_noalloc(C.byref(params), C.c_void_p(phi_ptr), C.c_void_p(P_ptr), C.c_void_p(v_ptr),
+                                   C.c_void_p(vstar_prev_ptr), C.c_void_p(pstar_prev_ptr), int(n),
+                                   C.c_void_p(stream) if stream else None))
 
 
 def contour_radius(field, level: float = 0.5):

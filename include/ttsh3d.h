@@ -277,6 +277,29 @@ int ttsh_batch_usage(ttsh_batch* b, size_t* bytes_used, int* active);
 
 int ttsh_batch_synchronize(ttsh_batch* b);
 
+/* ---- stepping without allocation ----------------------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: steps one droplet whose state lives in caller-owned
+ * device memory, and allocates nothing.  All pointers are device pointers to
+ * FP32 arrays in the layouts above: phi (N^3), P and v (3*N^3) in/out;
+ * vstar_prev, pstar_prev (3*N^3 each) in: the predictors the first step's
+ * stale-halo divergence reads (the reference's initial v*, P*; pass copies
+ * of v and P for a fresh state), out: the last step's predictors, so
+ * consecutive calls continue one run exactly.  n steps as ttsh_step with
+ * the same kernels, tiles and results; stream is a hipStream_t (NULL: the
+ * null stream); the call returns when the steps are done.
+ * Every other buffer of the step lives in one device array of 32 MiB per
+ * GPU, a library-wide static held until the process ends: N is limited to
+ * 64 (TTSH_EINVAL beyond).  Calls on one GPU share that array and keep the
+ * last call's contents (the droplet's region is cleared at every call).
+ * Calls that overlapped on one GPU would race on it, so the library
+ * serialises them per device: a call waits for the previous one to return,
+ * and a call returns only when its steps are done. */
+int ttsh_step_noalloc(const ttsh_params* p, float* phi, float* P, float* v, float* vstar_prev,
+                      float* pstar_prev, int64_t n, void* stream);
+
 /* ---- misc ------------------------------------------------------------------------ */
 
 /* Message of the last failed call on this thread ("" if none). */
