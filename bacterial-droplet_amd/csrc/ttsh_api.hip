@@ -105,6 +105,8 @@ struct Slot {
     double T = 0.0;         // T accumulated in double as the reference (TTSH3D.cu:342)
     int rhs_tz = 16, proj_tz = 8;
     bool rhs_tz_set = false;  // rhs_tz came from the parameters, not the tuning table
+    int64_t fin_call = -1;    // the step call that took the droplet's last step (limit reached)
+    bool published = false;   // in the device tables since its allocation
 };
 
 struct Table {
@@ -151,6 +153,70 @@ struct Engine {
     int engine_parity = 0;
     int last_chunk = -1;
     bool tables_dirty = true;
+
+    // Side stream for uploads, and for reads of droplets that reached their
+    // step limit: it never waits for the step kernels on st except through
+    // events.  Uploads are published to the step kernels at the next table
+    // rebuild, which makes st wait for up_ev.
+    hipStream_t us = nullptr;
+    hipEvent_t up_ev = nullptr, st_ev = nullptr;
+    bool up_pending = false;
+    // One event per step call (ttsh_batch_step / ttsh_step), recorded on st
+    // after the call's launches, in a ring.
+    static constexpr int kCallEv = 16;
+    hipEvent_t call_ev[kCallEv] = {};
+    int64_t ncalls = 0;
+    // Device memory of retired droplets, kept for reuse by droplets of the
+    // same size; ev marks the end of the work queued before the retirement.
+    struct Pooled {
+        int N;
+        void* mem;
+        size_t bytes;
+        hipEvent_t ev;
+    };
+    std::vector<Pooled> pool;
+    size_t pooled = 0;
+    Drop* d_rdrop = nullptr;  // descriptor of the droplet being reduced
+
+    hipStream_t side() const { return us ? us : st; }
+
+    // Work queued on the side stream so far (fills, uploads) is waited for by
+    // st at the next table rebuild, or before any other use on st.
+    hipError_t publish_side()
+    {
+        if (!us) return hipSuccess;
+        up_pending = true;
+        return hipEventRecord(up_ev, us);
+    }
+    hipError_t uploads_into_st()
+    {
+        return up_pending ? hipStreamWaitEvent(st, up_ev, 0) : hipSuccess;
+    }
+
+    // For a droplet that reached its step limit, the event after which no
+    // step kernel touches it (nullptr: none, or not recorded yet).
+    hipEvent_t fin_event(const Slot& s) const
+    {
+        if (!call_ev[0] || s.limit < 0 || s.steps < s.limit || s.fin_call < 0 || s.fin_call >= ncalls) return nullptr;
+        // a later call's event in the ring also marks the end of this one
+        const int64_t k = std::max<int64_t>(s.fin_call, ncalls - kCallEv);
+        return call_ev[k % kCallEv];
+    }
+
+    // Wait until every step call but the last `keep` is done on the device.
+    int wait_calls(int keep)
+    {
+        HIPCHK(hipSetDevice(device));
+        if (keep <= 0 || !call_ev[0]) {
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipStreamSynchronize(side()));
+            return TTSH_OK;
+        }
+        if (keep >= kCallEv) return fail(TTSH_EINVAL, "keep must be below " + std::to_string(kCallEv));
+        const int64_t k = ncalls - 1 - keep;
+        if (k >= 0) HIPCHK(hipEventSynchronize(call_ev[k % kCallEv]));
+        return TTSH_OK;
+    }
 
     // External arena (ttsh_step_noalloc): every device buffer is carved from
     // [arena, arena + arena_cap) instead of hipMalloc, nothing is freed, and
@@ -212,6 +278,22 @@ struct Engine {
         dev_free(d_cpart);
         dev_free(d_csum);
         dev_free(d_slice);
+        dev_free(d_rdrop);
+        d_rdrop = nullptr;
+        if (us) (void)hipStreamSynchronize(us);
+        for (auto& b : pool) {
+            dev_free(b.mem);
+            if (b.ev) (void)hipEventDestroy(b.ev);
+        }
+        pool.clear();
+        pooled = 0;
+        if (up_ev) (void)hipEventDestroy(up_ev);
+        if (st_ev) (void)hipEventDestroy(st_ev);
+        up_ev = st_ev = nullptr;
+        for (auto& e : call_ev)
+            if (e) { (void)hipEventDestroy(e); e = nullptr; }
+        if (us) (void)hipStreamDestroy(us);
+        us = nullptr;
         d_drops = nullptr; d_counts = nullptr; d_redpart = nullptr; d_redout = nullptr;
         d_cpart = nullptr; d_csum = nullptr; d_slice = nullptr; slice_N = 0;
         if (st && !external) (void)hipStreamDestroy(st);
@@ -231,8 +313,15 @@ struct Engine {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        if (!external) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        if (!external) {
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&up_ev, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&st_ev, hipEventDisableTiming));
+            for (auto& e : call_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         slots.resize(nslots);
+        HIPCHK(dev_alloc(&d_rdrop, sizeof(Drop)));
         HIPCHK(dev_alloc(&d_drops, sizeof(Drop) * nslots));
         h_drops.resize(nslots);
         HIPCHK(dev_alloc(&d_counts, sizeof(int) * kNTables));
@@ -261,6 +350,7 @@ struct Engine {
     int launch_diag(int si, double* sums, int z, float* slice)
     {
         Slot& sl = slots[si];
+        HIPCHK(uploads_into_st());
         const float* phi = sl.drop.phi[sl.q];
         HIPCHK(launch_centroid(phi, sl.N, d_cpart, kRedBlocks, sums, st));
         if (slice) {
@@ -394,10 +484,33 @@ struct Engine {
         }
         HIPCHK(hipSetDevice(device));
         void* mem = nullptr;
-        hipError_t e = dev_alloc(&mem, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(TTSH_ENOMEM, std::string("hipMalloc of droplet state failed: ") + hipGetErrorString(e));
+        // a retired droplet's memory of the same size, once the work queued
+        // before its retirement is done (the side stream waits for it)
+        for (size_t k = 0; k < pool.size(); ++k)
+            if (pool[k].N == N) {
+                mem = pool[k].mem;
+                HIPCHK(hipStreamWaitEvent(side(), pool[k].ev, 0));
+                HIPCHK(hipEventDestroy(pool[k].ev));
+                pooled -= pool[k].bytes;
+                pool.erase(pool.begin() + (long)k);
+                break;
+            }
+        if (!mem) {
+            if (budget && used + pooled + bytes > budget) {
+                int rc = release_pool();
+                if (rc) return rc;
+            }
+            hipError_t e = dev_alloc(&mem, bytes);
+            if (e != hipSuccess && !pool.empty()) {
+                (void)hipGetLastError();
+                int rc = release_pool();
+                if (rc) return rc;
+                e = dev_alloc(&mem, bytes);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(TTSH_ENOMEM, std::string("hipMalloc of droplet state failed: ") + hipGetErrorString(e));
+            }
         }
         if (v != rhs_variant) {
             rhs_variant = v;
@@ -443,11 +556,12 @@ struct Engine {
         D.eslot[1] = 1;
         D.faithful = p.faithful_halo ? 1 : 0;
         make_coef(p, D.co);
-        // zero state, sigma = 1 until the first step sets it (fill kernels:
-        // no memset node, should a caller capture an external engine)
-        HIPCHK(launch_fill_u32(mem, 0u, bytes / 4, st));
-        HIPCHK(launch_fill_u32(D.sigma, 0x3f800000u, 1, st));
-        HIPCHK(hipStreamSynchronize(st));
+        // zero state, sigma = 1 until the first step sets it, on the side
+        // stream (fill kernels: no memset node, should a caller capture an
+        // external engine); the upload that follows is ordered after them
+        HIPCHK(launch_fill_u32(mem, 0u, bytes / 4, side()));
+        HIPCHK(launch_fill_u32(D.sigma, 0x3f800000u, 1, side()));
+        HIPCHK(publish_side());
         s.q = engine_parity;
         s.steps = 0;
         s.limit = -1;
@@ -464,11 +578,36 @@ struct Engine {
         Slot& s = slots[si];
         if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
         HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamSynchronize(st));
-        dev_free(s.mem);
+        if (us) {
+            // keep the memory for the next droplet of this size; it may be
+            // reused once the work queued so far is done
+            hipEvent_t ev = nullptr;
+            HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(ev, st));
+            pool.push_back(Pooled{s.N, s.mem, s.bytes, ev});
+            pooled += s.bytes;
+        } else {
+            HIPCHK(hipStreamSynchronize(st));
+            dev_free(s.mem);
+        }
         used -= s.bytes;
         s = Slot();
         tables_dirty = true;
+        return TTSH_OK;
+    }
+
+    // Free the pooled memory of retired droplets (after the queued work).
+    int release_pool()
+    {
+        if (pool.empty()) return TTSH_OK;
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(side()));
+        for (auto& b : pool) {
+            dev_free(b.mem);
+            HIPCHK(hipEventDestroy(b.ev));
+        }
+        pool.clear();
+        pooled = 0;
         return TTSH_OK;
     }
 
@@ -522,6 +661,10 @@ struct Engine {
     // Rebuild descriptor and work tables from the active slots.
     int rebuild(int steps_this_call)
     {
+        // droplets filled or uploaded on the side stream join the step
+        // kernels from here on
+        HIPCHK(uploads_into_st());
+        up_pending = false;
         std::vector<Work> w[kNTables];
         const bool batch = batch_layout();
         const int rv = batch ? (int)RHS_32x8 : (rhs_variant < 0 ? 0 : rhs_variant);
@@ -536,6 +679,7 @@ struct Engine {
             Slot& s = slots[si];
             h_drops[si] = s.drop;
             if (!s.used) continue;
+            s.published = true;
             const int64_t rem = remaining(s);
             // a droplet without a step limit never stops inside a launch
             // sequence, so its descriptor serves any chunk length
@@ -724,6 +868,7 @@ struct Engine {
             if (!s.used) continue;
             const int64_t d = std::min<int64_t>(remaining(s), k);
             s.steps += d;
+            if (d > 0 && s.limit >= 0 && s.steps >= s.limit) s.fin_call = ncalls;
             for (int64_t i = 0; i < d; ++i) s.T = s.T + s.prm.dt0;  // T = T + Dt (TTSH3D.cu:342)
             if (d & 1) s.q ^= 1;
         }
@@ -733,6 +878,14 @@ struct Engine {
     int step(int64_t n, bool graphs)
     {
         HIPCHK(hipSetDevice(device));
+        const int rc = step_chunks(n, graphs);
+        if (call_ev[0]) HIPCHK(hipEventRecord(call_ev[ncalls % kCallEv], st));
+        ++ncalls;
+        return rc;
+    }
+
+    int step_chunks(int64_t n, bool graphs)
+    {
         while (n > 0) {
             if (!any_active()) return TTSH_OK;
             normalise_parity();
@@ -794,10 +947,14 @@ struct Engine {
             advance(m);
             n -= m;
         }
+        if (call_ev[0]) (void)hipEventRecord(call_ev[ncalls % kCallEv], st);
+        ++ncalls;
         for (auto& e : ev) (void)hipEventDestroy(e);
         return rc;
     }
 
+    // State in from host arrays, on the side stream: the step kernels queued
+    // on st keep running; the droplet joins them at the next rebuild.
     int upload_state(int si, int dtype, const void* phi, const void* P, const void* v, const void* vsp,
                      const void* psp)
     {
@@ -806,9 +963,14 @@ struct Engine {
         if (dtype != TTSH_F32 && dtype != TTSH_F64) return fail(TTSH_EINVAL, "dtype must be TTSH_F32 or TTSH_F64");
         if (!phi || !P || !v) return fail(TTSH_EINVAL, "phi, P and v are required");
         HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamSynchronize(st));
         normalise_parity();
         const size_t n = (size_t)s.N * s.N * s.N;
+        const hipStream_t ss = side();
+        if (s.published && us) {
+            // steps already queued may still use this droplet's buffers
+            HIPCHK(hipEventRecord(st_ev, st));
+            HIPCHK(hipStreamWaitEvent(ss, st_ev, 0));
+        }
         std::vector<float> tmp;
         auto put = [&](float* dst, const void* src, size_t off) -> int {
             const void* p = nullptr;
@@ -820,7 +982,8 @@ struct Engine {
                 for (size_t i = 0; i < n; ++i) tmp[i] = (float)d[i];
                 p = tmp.data();
             }
-            HIPCHK(hipMemcpy(dst, p, n * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(dst, p, n * sizeof(float), hipMemcpyHostToDevice, ss));
+            HIPCHK(hipStreamSynchronize(ss));  // the host buffer is reused or released next
             return TTSH_OK;
         };
         const int q = s.q;
@@ -834,10 +997,11 @@ struct Engine {
         {
             const float* vs[3] = {s.drop.vs[q ^ 1][0], s.drop.vs[q ^ 1][1], s.drop.vs[q ^ 1][2]};
             const float* ps[3] = {s.drop.ps[q ^ 1][0], s.drop.ps[q ^ 1][1], s.drop.ps[q ^ 1][2]};
-            HIPCHK(launch_fill_edges(vs, ps, s.drop.edge, s.drop.eslot[q ^ 1], s.N, st));
-            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(launch_fill_edges(vs, ps, s.drop.edge, s.drop.eslot[q ^ 1], s.N, ss));
+            HIPCHK(publish_side());
         }
         s.steps = 0;
+        s.fin_call = -1;
         s.T = s.prm.T0;
         tables_dirty = true;
         return TTSH_OK;
@@ -892,7 +1056,16 @@ struct Engine {
         if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
         if (dtype != TTSH_F32 && dtype != TTSH_F64) return fail(TTSH_EINVAL, "dtype must be TTSH_F32 or TTSH_F64");
         HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamSynchronize(st));
+        // a droplet past its step limit is read once its last step is done,
+        // without waiting for steps queued after it; otherwise everything
+        // queued so far completes first
+        if (hipEvent_t ev = fin_event(s)) {
+            HIPCHK(hipEventSynchronize(ev));
+        } else {
+            HIPCHK(uploads_into_st());
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        HIPCHK(hipStreamSynchronize(side()));
         const size_t n = (size_t)s.N * s.N * s.N;
         std::vector<float> tmp(n);
         auto get = [&](const float* src, void* dst, size_t off) -> int {
@@ -917,15 +1090,23 @@ struct Engine {
         Slot& s = slots[si];
         if (!s.used) return fail(TTSH_EINVAL, "slot not in use");
         HIPCHK(hipSetDevice(device));
+        // a droplet past its step limit is reduced on the side stream after
+        // its last step; otherwise on st after everything queued
+        hipStream_t rs = st;
+        if (hipEvent_t ev = fin_event(s)) {
+            rs = side();
+            HIPCHK(hipStreamWaitEvent(rs, ev, 0));
+        } else {
+            HIPCHK(uploads_into_st());
+        }
         Drop d = s.drop;
         // the reduce kernel reads phi[parity]; point it at the current buffer
         d.phi[0] = s.drop.phi[s.q];
-        HIPCHK(hipMemcpyAsync(d_drops + si, &d, sizeof(Drop), hipMemcpyHostToDevice, st));
-        HIPCHK(launch_reduce(d_drops + si, 1, 0, d_redpart, kRedBlocks, d_redout, st));
+        HIPCHK(hipMemcpyAsync(d_rdrop, &d, sizeof(Drop), hipMemcpyHostToDevice, rs));
+        HIPCHK(launch_reduce(d_rdrop, 1, 0, d_redpart, kRedBlocks, d_redout, rs));
         double raw[3];
-        HIPCHK(hipMemcpyAsync(raw, d_redout, sizeof(raw), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        tables_dirty = true;  // descriptor slot was overwritten
+        HIPCHK(hipMemcpyAsync(raw, d_redout, sizeof(raw), hipMemcpyDeviceToHost, rs));
+        HIPCHK(hipStreamSynchronize(rs));
         const double h = s.prm.h, h3 = h * h * h;
         const double n = (double)s.N * s.N * s.N;
         out[0] = raw[0] * h3;
@@ -995,6 +1176,7 @@ public:
         }
         const Slot& sl = e->slots[si];
         hipStream_t st = e->st;
+        HIPCHK(e->uploads_into_st());
         if (dsnap[b]) {
             const float* src[7] = {sl.drop.v[0], sl.drop.v[1], sl.drop.v[2], sl.drop.P[0],
                                    sl.drop.P[1], sl.drop.P[2], sl.drop.phi[sl.q]};
@@ -1253,9 +1435,7 @@ int ttsh_step(ttsh_sim* s, int64_t n)
 int ttsh_synchronize(ttsh_sim* s)
 {
     if (!s) return fail(TTSH_EINVAL, "null handle");
-    HIPCHK(hipSetDevice(s->eng.device));
-    HIPCHK(hipStreamSynchronize(s->eng.st));
-    return TTSH_OK;
+    return s->eng.wait_calls(0);
 }
 
 int ttsh_reductions(ttsh_sim* s, double out[3])
@@ -1288,6 +1468,7 @@ int ttsh_get_aux(ttsh_sim* s, int which, float* out)
     Slot& sl = e.slots[0];
     HIPCHK(hipSetDevice(e.device));
     HIPCHK(hipStreamSynchronize(e.st));
+    HIPCHK(hipStreamSynchronize(e.side()));
     const size_t n = (size_t)sl.N * sl.N * sl.N;
     const int last = sl.q ^ 1;  // parity of the last step's predictor
     if (which == 0 || which == 1) {
@@ -1500,8 +1681,28 @@ int ttsh_batch_usage(ttsh_batch* b, size_t* bytes_used, int* active)
 int ttsh_batch_synchronize(ttsh_batch* b)
 {
     if (!b) return fail(TTSH_EINVAL, "null handle");
+    return b->eng.wait_calls(0);
+}
+
+int ttsh_batch_wait(ttsh_batch* b, int keep)
+{
+    if (!b) return fail(TTSH_EINVAL, "null handle");
+    if (keep < 0) return fail(TTSH_EINVAL, "keep must be >= 0");
+    return b->eng.wait_calls(keep);
+}
+
+int ttsh_batch_ready(ttsh_batch* b, int slot, int* ready)
+{
+    int rc = check_slot(b, slot);
+    if (rc) return rc;
+    if (!ready) return fail(TTSH_EINVAL, "null output");
     HIPCHK(hipSetDevice(b->eng.device));
-    HIPCHK(hipStreamSynchronize(b->eng.st));
+    *ready = 0;
+    if (hipEvent_t ev = b->eng.fin_event(b->eng.slots[slot])) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) *ready = 1;
+        else if (e != hipErrorNotReady) return fail(TTSH_EHIP, std::string("hipEventQuery failed: ") + hipGetErrorString(e));
+    }
     return TTSH_OK;
 }
 

@@ -117,6 +117,8 @@ def lib():
         "ttsh_batch_retire": (C.c_int, [vp, C.c_int]),
         "ttsh_batch_usage": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
         "ttsh_batch_synchronize": (C.c_int, [vp]),
+        "ttsh_batch_wait": (C.c_int, [vp, C.c_int]),
+        "ttsh_batch_ready": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
         "ttsh_step_noalloc": (C.c_int, [P, vp, vp, vp, vp, vp, i64, vp]),
         "ttsh_tuning_reload": (C.c_int, [C.c_char_p]),
         "ttsh_tuning_lookup": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -427,6 +429,16 @@ class Batch:
         if rc not in (OK, ENONFINITE):
             _check(rc)
         return np.array(out[:]), rc == OK
+
+    def wait(self, keep: int = 0):
+        """Wait for every step call but the last `keep`."""
+        _check(lib().ttsh_batch_wait(self._h, int(keep)))
+
+    def ready(self, slot: int) -> bool:
+        """True when the droplet has reached its limit and those steps are done."""
+        r = C.c_int()
+        _check(lib().ttsh_batch_ready(self._h, int(slot), C.byref(r)))
+        return bool(r.value)
 
     def retire(self, slot: int):
         _check(lib().ttsh_batch_retire(self._h, int(slot)))

@@ -41,7 +41,8 @@ class Request:
     seed: int
     admitted: float = math.nan
     completed: float = math.nan
-    ok: bool = True
+    ok: bool = True          # finished with finite reductions
+    rejected: bool = False   # never admitted: does not fit an empty device
     reductions: tuple = ()
 
 
@@ -108,6 +109,12 @@ class GpuBackend:
     def steps_done(self, slot):
         return self.batch.steps_done(slot)
 
+    def wait(self, keep):
+        self.batch.wait(keep)
+
+    def ready(self, slot):
+        return self.batch.ready(slot)
+
     def finish(self, slot):
         red, ok = self.batch.reductions(slot)
         if self.keep_states:
@@ -125,41 +132,57 @@ class GpuBackend:
 
 
 class Server:
-    def __init__(self, backend, macro=16, clock=time.perf_counter, sleep=time.sleep):
+    """bypass: how many later arrivals may be admitted ahead of a head
+    request that does not fit yet (0 = strict FIFO); the head keeps its
+    place, and once it has been passed that often, admission waits for it."""
+
+    def __init__(self, backend, macro=16, clock=time.perf_counter, sleep=time.sleep, bypass=8):
         self.b = backend
         self.macro = macro
         self.clock = clock
         self.sleep = sleep
+        self.bypass = bypass
 
     def run(self, requests, log=None):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active = {}  # slot -> request
         done = []
+        passes = {}  # head request id -> later arrivals admitted ahead of it
         t0 = self.clock()
         busy_steps = 0
         for r in list(pending)[:16]:
             self.b.prepare(r)
         while pending or active:
             now = self.clock() - t0
-            # admit arrived requests in order while memory and slots allow
-            while pending and pending[0].arrival <= now:
-                r = pending[0]
+            # admit arrived requests in arrival order while memory and slots
+            # allow; later arrivals that fit may pass a head that does not,
+            # at most `bypass` times per head, after which admission waits
+            # for the head
+            i = 0
+            while i < len(pending) and pending[i].arrival <= now:
+                head = pending[0]
+                if i > 0 and passes.get(head.rid, 0) >= self.bypass:
+                    break
+                r = pending[i]
                 slot = self.b.admit(r)
                 if slot is None:
-                    if not active:
+                    if i == 0 and not active:
                         # cannot fit even on an empty device: reject it
                         pending.popleft()
                         r.ok = False
+                        r.rejected = True
                         r.admitted = r.completed = self.clock() - t0
                         done.append(r)
                         continue
-                    break
-                pending.popleft()
+                    i += 1
+                    continue
+                del pending[i]
+                if i > 0:
+                    passes[head.rid] = passes.get(head.rid, 0) + 1
                 r.admitted = self.clock() - t0
                 active[slot] = r
-                if len(pending) > 0:
-                    for q in list(pending)[:8]:
-                        self.b.prepare(q)
+                for q in list(pending)[:8]:
+                    self.b.prepare(q)
             if not active:
                 # idle until the next arrival
                 if pending:
@@ -167,11 +190,16 @@ class Server:
                     if wait > 0:
                         self.sleep(min(wait, 0.05))
                 continue
+            # queue the next macro step, then wait for the one before it: the
+            # device works on the new one while the host retires and admits
             self.b.step(self.macro)
             busy_steps += self.macro
-            self.b.sync()
+            if all(self.b.steps_done(slot) >= r.steps for slot, r in active.items()):
+                self.b.wait(0)  # nothing left running: wait for the last steps
+            else:
+                self.b.wait(1)
             for slot, r in list(active.items()):
-                if self.b.steps_done(slot) >= r.steps:
+                if self.b.ready(slot):
                     r.reductions, r.ok = self.b.finish(slot)
                     r.completed = self.clock() - t0
                     done.append(r)
@@ -183,14 +211,19 @@ class Server:
 
 
 def summarize(done, wall):
-    lat = np.array([r.completed - r.arrival for r in done])
-    queue = np.array([r.admitted - r.arrival for r in done])
-    cells = sum(r.N ** 3 * r.steps for r in done)
+    """Throughput and latency over the requests that ran to completion with
+    finite results; rejected requests (never admitted) and diverged ones
+    (non-finite reductions) are counted apart and enter neither."""
+    ok = [r for r in done if r.ok]
+    lat = np.array([r.completed - r.arrival for r in ok])
+    queue = np.array([r.admitted - r.arrival for r in ok])
+    cells = sum(r.N ** 3 * r.steps for r in ok)
     return {
-        "completed": len(done),
-        "failed": sum(1 for r in done if not r.ok),
+        "completed": len(ok),
+        "failed": sum(1 for r in done if not r.ok and not r.rejected),
+        "rejected": sum(1 for r in done if r.rejected),
         "wall_s": round(wall, 3),
-        "sims_per_hour": round(len(done) / wall * 3600.0, 2),
+        "sims_per_hour": round(len(ok) / wall * 3600.0, 2),
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
         "latency_p99_s": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
         "queue_p50_s": round(float(np.percentile(queue, 50)), 4) if len(queue) else None,

@@ -17,8 +17,12 @@
  *   - A ttsh_sim or ttsh_batch owns its device memory and one HIP stream.
  *     Handles are not thread-safe; use one handle per thread.
  *   - Grids are cubic (Nx = Ny = Nz = N, as the reference's cuFFT plan
- *     requires, TTSH3D.cu:1123), N a multiple of 16 in [32, 256] whose only
- *     prime factors are 2 and 3.
+ *     requires, TTSH3D.cu:1123) with N in {32, 48, 64, 96, 128, 144, 192,
+ *     256}: the sizes with a compile-time FFT plan (prime factors 2 and 3).
+ *     The reference's default and swept N is 64 (run_cuda.pl:14-16,
+ *     forEach.pl:30); any other N its cuFFT plan accepts -- below 32, above
+ *     256, or with a prime factor of 5 or more (e.g. 16, 40, 80, 100, 160)
+ *     -- is refused here with TTSH_EUNSUPPORTED.
  */
 #ifndef TTSH3D_H
 #define TTSH3D_H
@@ -276,6 +280,20 @@ int ttsh_batch_retire(ttsh_batch* b, int slot);
 int ttsh_batch_usage(ttsh_batch* b, size_t* bytes_used, int* active);
 
 int ttsh_batch_synchronize(ttsh_batch* b);
+
+/* New: wait until every ttsh_batch_step call but the last `keep` (0 <= keep
+ * < 16) is done on the device; keep = 0 is ttsh_batch_synchronize.  A
+ * request loop queues the next steps and then waits for the previous ones,
+ * so the device never idles while the host retires and admits droplets. */
+int ttsh_batch_wait(ttsh_batch* b, int keep);
+
+/* New: *ready = 1 when the droplet has reached its step limit and those
+ * steps are done on the device.  Its state and reductions are then read
+ * after its own last step, without waiting for steps queued later (which do
+ * not touch it).  Admission likewise uploads on a second stream without
+ * waiting for queued steps; the droplet joins them at the next step call,
+ * and a retired droplet's memory is kept for the next droplet of its size. */
+int ttsh_batch_ready(ttsh_batch* b, int slot, int* ready);
 
 /* ---- stepping without allocation ----------------------------------------------- */
 

@@ -158,3 +158,46 @@ def test_layout_switch_mid_run_equals_alone(ttsh):
         ref = alone(ttsh, N, cA, La, seed, lim)
         for a, c in zip(ref, g):
             assert np.array_equal(a, c), f"{N}^3 droplet seed {seed} ({lim} steps) differs between batch and alone"
+
+
+def test_ready_wait_and_admission_while_steps_run(ttsh):
+    """ttsh_batch_ready turns true once a droplet's limit is reached and those
+    steps are done; its state then stays untouched by later steps.  A droplet
+    admitted while steps are queued (uploaded on the side stream, published
+    at the next step call) and a retired droplet's reused memory both give
+    the run-alone result, bitwise."""
+    log("[ready_wait_and_admission_while_steps_run]")
+    f = lambda a: a.astype(np.float32)
+    pa = ttsh.default_params(N=48, dt0=0.01, cA=-1.5)
+    pb = ttsh.default_params(N=32, dt0=0.01)
+    pc = ttsh.default_params(N=48, dt0=0.01, La=1.2)
+    sa, sb, sc = (ttsh.init_droplet(48, 14.4, 2.0, 0, 91), ttsh.init_droplet(32, 9.6, 2.0, 0, 92),
+                  ttsh.init_droplet(48, 14.4, 2.0, 0, 93))
+    with ttsh.Batch(max_slots=4, use_graph=True) as b:
+        a = b.admit(pa, *map(f, sa), dtype=ttsh.F32, limit=20)
+        bb = b.admit(pb, *map(f, sb), dtype=ttsh.F32)
+        b.step(32)
+        b.wait(0)
+        assert b.ready(a) and not b.ready(bb)
+        got_a = b.get_state(a)
+        b.step(64)           # queued; droplet a is past its limit
+        assert b.ready(a)
+        assert all(np.array_equal(x, y) for x, y in zip(got_a, b.get_state(a)))
+        b.retire(a)          # its memory is kept for the next 48^3 droplet
+        c = b.admit(pc, *map(f, sc), dtype=ttsh.F32, limit=24)  # admitted while steps run
+        b.step(32)
+        b.wait(1)
+        b.synchronize()
+        assert b.ready(c) and b.steps_done(c) == 24 and b.steps_done(bb) == 128
+        got_c = b.get_state(c)
+        got_b = b.get_state(bb)
+    for got, n, p, st in ((got_a, 20, pa, sa), (got_c, 24, pc, sc), (got_b, 128, pb, sb)):
+        for x, y in zip(alone_f32(ttsh, p, st, n), got):
+            assert np.array_equal(x, y), f"N={p.Nx}, {n} steps"
+
+
+def alone_f32(ttsh, p, st, steps):
+    with ttsh.Simulation(p) as sim:
+        sim.set_state(*[x.astype(np.float32) for x in st], dtype=ttsh.F32)
+        sim.step(steps)
+        return sim.get_state()

@@ -51,6 +51,13 @@ class FakeBackend:
     def sync(self):
         pass
 
+    def wait(self, keep):
+        pass
+
+    def ready(self, slot):
+        r, steps, _ = self.slots[slot]
+        return steps >= r.steps
+
     def steps_done(self, slot):
         return self.slots[slot][1]
 
@@ -115,3 +122,36 @@ def test_request_generator_is_seeded():
     assert [(r.arrival, r.N, r.steps, r.cA, r.La) for r in a] == [(r.arrival, r.N, r.steps, r.cA, r.La) for r in b]
     assert all(64 <= r.N <= 192 and 1000 <= r.steps <= 20000 for r in a)
     assert all(-10.0 <= r.cA <= -0.1 and 0.5 <= r.La ** 2 <= 2.0 for r in a)
+
+
+def test_bounded_bypass_of_a_blocked_head():
+    """A head request that does not fit lets later arrivals that fit pass it,
+    at most `bypass` times; then admission waits for the head."""
+    clock = FakeClock()
+    big = lambda i: serve.Request(i, 0.0, 64, 640, -1.0, 1.0, i)
+    small = lambda i: serve.Request(i, 0.0, 32, 64, -1.0, 1.0, i)
+    reqs = [big(0), big(1), big(2), small(3), small(4), small(5)]
+    budget = 88 * (2 * 64 ** 3 + 2 * 32 ** 3)
+    for bypass, passed in ((0, []), (1, [3]), (8, [3, 4, 5])):
+        clock.t = 0.0
+        be = FakeBackend(clock, budget, max_slots=8)
+        serve.Server(be, macro=16, clock=clock, sleep=clock.sleep, bypass=bypass).run([
+            serve.Request(r.rid, r.arrival, r.N, r.steps, r.cA, r.La, r.seed) for r in reqs])
+        order = [rid for rid, _, _ in be.admit_log]
+        head_at = order.index(2)
+        assert order[:2] == [0, 1]
+        assert sorted(order[2:head_at]) == passed, (bypass, order)
+
+
+def test_summary_leaves_out_rejected_requests():
+    """A rejected request (never stepped) counts as rejected only: it adds no
+    cells, no completion and no latency."""
+    clock = FakeClock()
+    reqs = [serve.Request(0, 0.0, 256, 32, -1.0, 1.0, 0), serve.Request(1, 0.0, 32, 32, -1.0, 1.0, 1)]
+    be = FakeBackend(clock, budget=88 * 64 ** 3, max_slots=4)
+    done, wall = serve.Server(be, macro=16, clock=clock, sleep=clock.sleep).run(reqs)
+    s = serve.summarize(done, wall)
+    assert s["completed"] == 1 and s["rejected"] == 1 and s["failed"] == 0
+    ok = next(r for r in done if r.rid == 1)
+    assert s["latency_p50_s"] == pytest.approx(ok.completed - ok.arrival, abs=1e-4)
+    assert s["mcell_steps_per_s"] == pytest.approx(32 ** 3 * 32 / wall / 1e6, rel=1e-2)
