@@ -546,6 +546,18 @@ hipError_t launch_rhs(int variant, const Drop* drops, const Work* work, const in
     }
 }
 
+// Work-groups per CU that the LDS of a tile allows (160 KiB per CU): the
+// phi/P planes over a 2-cell apron (13 floats a position), v and the
+// intermediates over a 1-cell apron (9 and 13).
+int rhs_groups_per_cu(int variant)
+{
+    int tx, ty;
+    rhs_tile(variant, &tx, &ty);
+    if (tx <= 0) return 1;
+    const int bytes = 4 * ((tx + 4) * (ty + 4) * 13 + (tx + 2) * (ty + 2) * (9 + 13));
+    return std::max(1, (160 * 1024) / bytes);
+}
+
 void rhs_tile(int variant, int* tx, int* ty)
 {
     if (variant >= 0 && variant < RHS_NVARIANTS) {

@@ -633,29 +633,53 @@ struct Engine {
     // Layout of a batch that fills the device.  The tuned tile and z-depth of
     // a size are chosen for that droplet alone, with enough z-chunks to fill
     // the device; each chunk re-reads two warm-up planes.  A batch that fills
-    // the device by itself runs K_rhs with the 32x8 tile (three groups per CU) and
-    // chunks of at least kBatchRhsTz planes, which cuts the warm-up share and
-    // gives items of nearly equal cost.  It applies when at least two
-    // droplets are active, none asks for a tile, every one fits 32x8, and the
-    // items make at least two rounds of resident groups.  K_rhs has no
+    // the device by itself runs every droplet's K_rhs with one tile (the
+    // batch entry of the tuning table; by default 32x8, three groups per CU)
+    // and chunks of at least batch_rhs_tz planes, which cuts the warm-up share
+    // and gives items of nearly equal cost.  It applies when at least two
+    // droplets are active, none asks for a tile, every one fits the tile, and
+    // the items make at least two rounds of resident groups.  K_rhs has no
     // reductions, so a droplet's result does not depend on this layout.
-    static constexpr int kBatchRhsTz = 32;
-    // K_corr (no reductions either) and K_div take chunks of about this depth
-    // in the same batches.  K_div's sums, which set the k = 0 coefficient and
-    // sigma, stay per tuned chunk (Drop::part), so they do not change either.
-    static constexpr int kBatchCorrTz = 32;
+    // K_corr (no reductions either) and K_div take chunks of about
+    // batch_corr_tz planes in the same batches.  K_div's sums, which set the
+    // k = 0 coefficient and sigma, stay per tuned chunk (Drop::part), so they
+    // do not change either.
+    int batch_rhs_variant = RHS_32x8, batch_rhs_tz = 32, batch_corr_tz = 32;
+    int tune_gen_seen = -1;
+
+    // Re-read the batch entry of the tuning table after a reload; a running
+    // batch is laid out again at its next rebuild.
+    void refresh_tuning()
+    {
+        const int g = tune_generation();
+        if (g == tune_gen_seen) return;
+        tune_gen_seen = g;
+        int v = RHS_32x8, rtz = 32, ctz = 32;
+        if (!tune_batch_lookup(&v, &rtz, &ctz) || v < 0 || v >= RHS_NVARIANTS) {
+            v = RHS_32x8;
+            rtz = 32;
+            ctz = 32;
+        }
+        batch_rhs_variant = v;
+        batch_rhs_tz = std::max(1, rtz);
+        batch_corr_tz = std::max(1, ctz);
+        tables_dirty = true;
+    }
+
     bool batch_layout() const
     {
         int active = 0;
         int64_t items = 0;
+        int tx, ty;
+        rhs_tile(batch_rhs_variant, &tx, &ty);
         for (const Slot& s : slots) {
             if (!s.used || remaining(s) <= 0) continue;
-            if (s.prm.rhs_variant >= 0 || !variant_fits(RHS_32x8, s.N)) return false;
+            if (s.prm.rhs_variant >= 0 || !variant_fits(batch_rhs_variant, s.N)) return false;
             ++active;
-            const int tz = std::min(std::max(s.rhs_tz, kBatchRhsTz), s.N);
-            items += (int64_t)((s.N + 31) / 32) * ((s.N + 7) / 8) * ((s.N + tz - 1) / tz);
+            const int tz = std::min(std::max(s.rhs_tz, batch_rhs_tz), s.N);
+            items += (int64_t)((s.N + tx - 1) / tx) * ((s.N + ty - 1) / ty) * ((s.N + tz - 1) / tz);
         }
-        return active >= 2 && items >= 2 * 3 * (int64_t)cus;
+        return active >= 2 && items >= 2 * (int64_t)rhs_groups_per_cu(batch_rhs_variant) * cus;
     }
 
     // Rebuild descriptor and work tables from the active slots.
@@ -667,7 +691,7 @@ struct Engine {
         up_pending = false;
         std::vector<Work> w[kNTables];
         const bool batch = batch_layout();
-        const int rv = batch ? (int)RHS_32x8 : (rhs_variant < 0 ? 0 : rhs_variant);
+        const int rv = batch ? batch_rhs_variant : (rhs_variant < 0 ? 0 : rhs_variant);
         if (rv != rhs_active) {
             HIPCHK(hipStreamSynchronize(st));
             rhs_active = rv;
@@ -687,7 +711,7 @@ struct Engine {
             if (rem <= 0) continue;
             const int N = s.N;
             // z is split into ceil(N/tz) chunks of near-equal depth
-            const int rtz = batch && !s.rhs_tz_set ? std::min(std::max(s.rhs_tz, kBatchRhsTz), N) : s.rhs_tz;
+            const int rtz = batch && !s.rhs_tz_set ? std::min(std::max(s.rhs_tz, batch_rhs_tz), N) : s.rhs_tz;
             const int nzr = (N + rtz - 1) / rtz;
             for (int k = 0; k < nzr; ++k) {
                 const int zb = k * N / nzr, ze = (k + 1) * N / nzr;
@@ -700,13 +724,13 @@ struct Engine {
             // (one, except in a device-filling batch)
             const int ntile = ((N + kProjTX - 1) / kProjTX) * ((N + kProjTY - 1) / kProjTY);
             const int nzp = (N + s.proj_tz - 1) / s.proj_tz;
-            const int g = batch && s.prm.corr_tz <= 0 ? std::max(1, std::min(nzp, kBatchCorrTz * nzp / N)) : 1;
+            const int g = batch && s.prm.corr_tz <= 0 ? std::max(1, std::min(nzp, batch_corr_tz * nzp / N)) : 1;
             for (int k0 = 0; k0 < nzp; k0 += g) {
                 const int k1 = std::min(nzp, k0 + g);
                 const int zb = k0 * N / nzp, ze = k1 * N / nzp;
                 for (int t = 0; t < ntile; ++t) w[T_PROJ].push_back(Work{si, t, zb | (ze << 16), k0 * ntile + t});
             }
-            const int ctz = batch && s.prm.corr_tz <= 0 ? std::min(std::max(s.proj_tz, kBatchCorrTz), N) : s.proj_tz;
+            const int ctz = batch && s.prm.corr_tz <= 0 ? std::min(std::max(s.proj_tz, batch_corr_tz), N) : s.proj_tz;
             const int nzc = (N + ctz - 1) / ctz;
             for (int k = 0; k < nzc; ++k) {
                 const int zb = k * N / nzc, ze = (k + 1) * N / nzc;
@@ -878,6 +902,7 @@ struct Engine {
     int step(int64_t n, bool graphs)
     {
         HIPCHK(hipSetDevice(device));
+        refresh_tuning();
         const int rc = step_chunks(n, graphs);
         if (call_ev[0]) HIPCHK(hipEventRecord(call_ev[ncalls % kCallEv], st));
         ++ncalls;
@@ -922,6 +947,7 @@ struct Engine {
     int profile(int64_t n, double ms[KK_N])
     {
         HIPCHK(hipSetDevice(device));
+        refresh_tuning();
         for (int k = 0; k < KK_N; ++k) ms[k] = 0.0;
         constexpr int64_t kChunk = 256;
         constexpr int E = KK_N + 1;

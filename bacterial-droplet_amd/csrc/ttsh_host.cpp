@@ -3,6 +3,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -64,16 +65,21 @@ bool supported_N(int N, std::string* why)
 
 // ---------------------------------------------------------------------------
 // Tuning table: per grid size N, the K_rhs tile variant, its z-depth and the
-// z-depth of the divergence/corrector stencils.  Written by the autotuner
-// (ttsh3d/autotune.py) as text lines "N rhs_variant rhs_tz proj_tz"; read from
+// z-depth of the divergence/corrector stencils, as text lines
+// "N rhs_variant rhs_tz proj_tz"; and the layout of a batch that fills the
+// device, a line "batch rhs_variant rhs_tz proj_tz" (the K_rhs tile every
+// droplet then runs, the minimum K_rhs z-depth, the divergence/corrector
+// z-depth).  Written by the autotuner (ttsh3d/autotune.py); read from
 // $TTSH_TUNE_FILE or tuned.txt beside libttsh3d.so.
 struct TuneEntry {
     int rhs_variant, rhs_tz, proj_tz;
 };
 static std::mutex g_tune_mu;
 static std::map<int, TuneEntry> g_tune;
+static TuneEntry g_tune_batch{-1, 0, 0};
 static bool g_tune_loaded = false;
 static std::string g_tune_path;
+static std::atomic<int> g_tune_gen{0};
 
 static std::string default_tune_path()
 {
@@ -91,8 +97,10 @@ static std::string default_tune_path()
 static int load_tune_locked(const std::string& path)
 {
     g_tune.clear();
+    g_tune_batch = TuneEntry{-1, 0, 0};
     g_tune_loaded = true;
     g_tune_path = path;
+    g_tune_gen.fetch_add(1);
     std::ifstream in(path);
     if (!in.is_open()) return TTSH_EIO;
     std::string line;
@@ -100,11 +108,33 @@ static int load_tune_locked(const std::string& path)
         const size_t h = line.find('#');
         if (h != std::string::npos) line.resize(h);
         std::istringstream ss(line);
-        int N, v, rtz, ptz;
-        if (ss >> N >> v >> rtz >> ptz) g_tune[N] = TuneEntry{v, rtz, ptz};
+        std::string key;
+        if (!(ss >> key)) continue;
+        int v, rtz, ptz;
+        if (!(ss >> v >> rtz >> ptz)) continue;
+        if (key == "batch") {
+            if (v >= 0 && rtz > 0 && ptz > 0) g_tune_batch = TuneEntry{v, rtz, ptz};
+        } else {
+            char* end = nullptr;
+            const long N = std::strtol(key.c_str(), &end, 10);
+            if (end && *end == 0 && N > 0) g_tune[(int)N] = TuneEntry{v, rtz, ptz};
+        }
     }
     return TTSH_OK;
 }
+
+bool tune_batch_lookup(int* rhs_variant, int* rhs_tz, int* proj_tz)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_loaded) (void)load_tune_locked(default_tune_path());
+    if (g_tune_batch.rhs_variant < 0) return false;
+    *rhs_variant = g_tune_batch.rhs_variant;
+    *rhs_tz = g_tune_batch.rhs_tz;
+    *proj_tz = g_tune_batch.proj_tz;
+    return true;
+}
+
+int tune_generation() { return g_tune_gen.load(); }
 
 bool tune_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz)
 {
@@ -156,6 +186,13 @@ int ttsh_tuning_reload(const char* path)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     const int rc = load_tune_locked(path ? std::string(path) : default_tune_path());
     return rc == TTSH_OK ? TTSH_OK : fail(rc, "cannot read tuning table " + g_tune_path);
+}
+
+int ttsh_tuning_lookup_batch(int* rhs_variant, int* rhs_tz, int* proj_tz)
+{
+    if (!rhs_variant || !rhs_tz || !proj_tz) return fail(TTSH_EINVAL, "null argument");
+    if (!tune_batch_lookup(rhs_variant, rhs_tz, proj_tz)) return fail(TTSH_EINVAL, "no batch entry in the tuning table");
+    return TTSH_OK;
 }
 
 int ttsh_tuning_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz)

@@ -17,5 +17,10 @@ int write_conf_bin_host(const char* path, int N, int m, double T, int64_t steps,
 int contour_radius(int N, const float* field, double level, double* R, double* center);
 // Tuned K_rhs variant, K_rhs z-depth and projection z-depth for size N.
 bool tune_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz);
+// Tuned layout of a batch that fills the device: K_rhs variant, minimum
+// K_rhs z-depth, projection z-depth (false: no entry).
+bool tune_batch_lookup(int* rhs_variant, int* rhs_tz, int* proj_tz);
+// Incremented whenever the tuning table is (re)loaded.
+int tune_generation();
 
 }  // namespace ttsh

@@ -27,6 +27,7 @@ enum { FFT_MODE_FWD = 0, FFT_MODE_INV = 1, FFT_MODE_POISSON = 2 };
 hipError_t launch_rhs(int variant, const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep,
                       int grid, hipStream_t st);
 void rhs_tile(int variant, int* tx, int* ty);
+int rhs_groups_per_cu(int variant);
 
 // s = (D.v* + i D.P*)/dt0 with the stale-halo rule; FP64 partial sums of s.
 hipError_t launch_div(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep, int grid,

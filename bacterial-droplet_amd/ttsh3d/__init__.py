@@ -122,6 +122,7 @@ def lib():
         "ttsh_step_noalloc": (C.c_int, [P, vp, vp, vp, vp, vp, i64, vp]),
         "ttsh_tuning_reload": (C.c_int, [C.c_char_p]),
         "ttsh_tuning_lookup": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "ttsh_tuning_lookup_batch": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "ttsh_last_error": (C.c_char_p, []),
         "ttsh_version": (C.c_char_p, []),
     }
@@ -201,6 +202,27 @@ def init_droplet(N: int, R: float, xi: float, pmode: int = 0, seed: int = 0):
     _check(lib().ttsh_init_droplet(int(N), float(R), float(xi), int(pmode), C.c_uint64(seed),
                                    _ptr(phi), _ptr(P), _ptr(v)))
     return phi, P, v
+
+
+def tuning_reload(path: str | None = None) -> None:
+    """Re-read the tuning table (None: $TTSH_TUNE_FILE or tuned.txt beside the library)."""
+    _check(lib().ttsh_tuning_reload(path.encode() if path else None))
+
+
+def tuning_lookup(N: int):
+    """(rhs_variant, rhs_tz, corr_tz) of size N, or None without an entry."""
+    v, a, b = C.c_int(), C.c_int(), C.c_int()
+    if lib().ttsh_tuning_lookup(int(N), C.byref(v), C.byref(a), C.byref(b)) != OK:
+        return None
+    return v.value, a.value, b.value
+
+
+def tuning_lookup_batch():
+    """(rhs_variant, rhs_tz, corr_tz) of a device-filling batch, or None without an entry."""
+    v, a, b = C.c_int(), C.c_int(), C.c_int()
+    if lib().ttsh_tuning_lookup_batch(C.byref(v), C.byref(a), C.byref(b)) != OK:
+        return None
+    return v.value, a.value, b.value
 
 
 def step_noalloc(params: "Params", phi_ptr: int, P_ptr: int, v_ptr: int, vstar_prev_ptr: int, pstar_prev_ptr: int,

@@ -6,7 +6,16 @@ the divergence/corrector stencils, on a droplet of that size; then writes the
 fastest choices as the tuning table libttsh3d.so reads (tuned.txt beside the
 library, lines "N rhs_variant rhs_tz proj_tz").
 
+With --batch it tunes the layout of a batch that fills the device instead:
+on the 64-droplet sweep mix (L in {48, 64, 96, 128}) as one graph batch, it
+times whole batch steps for every K_rhs tile that fits all sizes, minimum
+K_rhs z-depth and divergence/corrector z-depth, through the table's "batch"
+line (re-read between candidates; the batch is laid out again at its next
+step), and keeps the fastest.  The FFT passes' lines per work-group are
+compile-time constants and not part of the search.
+
     python -m ttsh3d.autotune [--sizes 48 64 96 128 192 256] [--steps 30] [--out PATH]
+    python -m ttsh3d.autotune --batch [--steps 48] [--out PATH]
 """
 from __future__ import annotations
 
@@ -16,7 +25,8 @@ import os
 import sys
 import time
 
-from . import F64, TTSHError, Simulation, default_params, init_droplet, lib, LIB_PATH
+from . import (F64, TTSHError, Batch, Simulation, default_params, init_droplet, lib, LIB_PATH, tuning_lookup,
+               tuning_lookup_batch, tuning_reload)
 
 RHS_VARIANTS = {0: (32, 8), 1: (16, 16), 2: (64, 4), 3: (16, 8), 4: (32, 8), 5: (32, 16), 6: (64, 8)}
 DEFAULT_TABLE = os.path.join(os.path.dirname(LIB_PATH), "tuned.txt")
@@ -62,12 +72,72 @@ def tune_size(N: int, steps: int = 30, log=print):
     return (best_rhs[0], best_rhs[1], best_proj[0]), results
 
 
-def write_table(entries: dict, path: str):
+def read_table(path: str):
+    """(per-size entries, batch entry or None) of a tuning table file."""
+    entries, batch = {}, None
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                t = line.split("#")[0].split()
+                if len(t) < 4:
+                    continue
+                if t[0] == "batch":
+                    batch = tuple(int(x) for x in t[1:4])
+                else:
+                    entries[int(t[0])] = tuple(int(x) for x in t[1:4])
+    return entries, batch
+
+
+def write_table(entries: dict, path: str, batch=None):
     with open(path, "w") as f:
         f.write("# N rhs_variant rhs_tz proj_tz  (written by ttsh3d.autotune)\n")
         for N in sorted(entries):
             v, rtz, ptz = entries[N]
             f.write(f"{N} {v} {rtz} {ptz}\n")
+        if batch is not None:
+            f.write("# device-filling batch: rhs_variant min_rhs_tz proj_tz\n")
+            f.write("batch {} {} {}\n".format(*batch))
+
+
+def tune_batch(table: str, steps: int = 48, log=print):
+    """Fastest device-filling batch layout on the 64-droplet sweep mix.
+    Candidates are written one at a time into a scratch copy of the table
+    (its per-size lines kept) and re-read; returns ((variant, rhs_tz,
+    corr_tz), ms per batch step, all results)."""
+    from . import workloads
+    import math
+    entries, _ = read_table(table)
+    cases = workloads.sweep_mix(__import__("ttsh3d"))
+    sizes = sorted({L for _, L, _ in cases})
+    variants = [v for v, (tx, ty) in RHS_VARIANTS.items() if all(L >= tx + 2 and L >= ty + 2 for L in sizes)]
+    scratch = table + ".tuning"
+    results = []
+    best = None
+    b = Batch(max_slots=len(cases), use_graph=True)
+    try:
+        for p, L, s in cases:
+            b.admit(p, *workloads.mix_state(__import__("ttsh3d"), L, s))
+        for var in variants:
+            for rtz in (16, 24, 32, 48, 64):
+                for ctz in (16, 32, 64):
+                    write_table(entries, scratch, (var, rtz, ctz))
+                    tuning_reload(scratch)
+                    b.step(16)  # re-layout and graph capture
+                    b.synchronize()
+                    t0 = time.perf_counter()
+                    b.step(steps)
+                    b.synchronize()
+                    ms = (time.perf_counter() - t0) * 1e3 / steps
+                    results.append({"variant": var, "rhs_tz": rtz, "corr_tz": ctz, "ms": ms})
+                    log(f"batch variant {var} {RHS_VARIANTS[var]} rhs_tz {rtz} corr_tz {ctz}: {ms:.3f} ms/step")
+                    if best is None or ms < best[1]:
+                        best = ((var, rtz, ctz), ms)
+    finally:
+        b.close()
+        if os.path.exists(scratch):
+            os.remove(scratch)
+        tuning_reload(table if os.path.exists(table) else None)
+    return best[0], best[1], results
 
 
 def main(argv=None):
@@ -76,8 +146,22 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--json", default=None, help="also write all timings here")
+    ap.add_argument("--batch", action="store_true", help="tune the device-filling batch layout instead")
     a = ap.parse_args(argv)
     lib()
+    if a.batch:
+        t0 = time.time()
+        entries, _ = read_table(a.out)
+        best, ms, res = tune_batch(a.out, a.steps if a.steps != 30 else 48,
+                                   log=lambda *x: print(*x, flush=True))
+        write_table(entries, a.out, best)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump({"batch": res, "best": best, "ms": ms}, f, indent=1)
+        print(f"batch layout: variant {best[0]} rhs_tz {best[1]} corr_tz {best[2]} ({ms:.3f} ms per batch step); "
+              f"wrote {a.out} in {time.time() - t0:.1f} s", flush=True)
+        tuning_reload(a.out)
+        return 0
     entries, allres = {}, {}
     t0 = time.time()
     for N in a.sizes:
@@ -85,7 +169,8 @@ def main(argv=None):
         entries[N] = best
         allres[N] = res
         print(f"N={N}: best rhs variant {best[0]} tz {best[1]}, proj tz {best[2]}", flush=True)
-    write_table(entries, a.out)
+    _, batch = read_table(a.out)
+    write_table(entries, a.out, batch)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(allres, f, indent=1)

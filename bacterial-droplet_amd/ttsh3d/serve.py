@@ -257,25 +257,37 @@ def main(argv=None):
     ap.add_argument("--sizes", type=int, nargs="+", default=[64, 96, 128, 192])
     ap.add_argument("--steps", type=int, nargs=2, default=[1000, 20000])
     ap.add_argument("--rate-frac", type=float, default=0.8, help="arrival rate as a fraction of capacity")
+    ap.add_argument("--rate", type=float, default=None,
+                    help="arrival rate per second (default: rate-frac of the capacity measured first)")
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--max-slots", type=int, default=64)
     ap.add_argument("--macro", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1005)
+    ap.add_argument("--part", type=int, nargs=2, default=None, metavar=("FIRST", "COUNT"),
+                    help="serve requests FIRST .. FIRST+COUNT-1 of the stream, arrivals shifted to start at "
+                         "the first one's (a stream longer than one run can hold is served in parts)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     budget = int(a.budget_gib * (1 << 30))
     cap = measure_capacity(a.sizes, a.steps, budget, a.max_slots)
     mean_work = np.mean([N ** 3 for N in a.sizes]) * (a.steps[0] + a.steps[1]) / 2.0
-    rate = a.rate_frac * cap / mean_work
+    rate = a.rate if a.rate else a.rate_frac * cap / mean_work
     print(f"capacity {cap / 1e6:.0f} Mcell-steps/s; arrival rate {rate:.3f}/s", flush=True)
     reqs = make_requests(a.requests, rate, tuple(a.sizes), tuple(a.steps), a.seed)
+    if a.part:
+        first, count = a.part
+        reqs = reqs[first:first + count]
+        t0 = reqs[0].arrival
+        for r in reqs:
+            r.arrival -= t0
     be = GpuBackend(a.max_slots, budget)
     srv = Server(be, macro=a.macro)
     done, wall = srv.run(reqs, log=lambda m: print(m, flush=True))
     be.close()
     out = summarize(done, wall)
     out.update({"requests": a.requests, "rate_per_s": rate, "capacity_mcell_steps_per_s": round(cap / 1e6, 1),
-                "sizes": a.sizes, "steps": a.steps, "rate_frac": a.rate_frac})
+                "sizes": a.sizes, "steps": a.steps, "rate_frac": a.rate_frac, "part": a.part,
+                "latency_s": {str(r.rid): round(r.completed - r.arrival, 4) for r in done if r.ok}})
     print(json.dumps(out), flush=True)
     if a.json:
         with open(a.json, "w") as f:

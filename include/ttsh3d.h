@@ -327,11 +327,18 @@ const char* ttsh_last_error(void);
 const char* ttsh_version(void);
 
 /* New: the tuning table the autotuner writes (lines "N rhs_variant rhs_tz
- * corr_tz"), read from $TTSH_TUNE_FILE or tuned.txt beside libttsh3d.so and
- * applied when ttsh_params leaves rhs_variant = -1, rhs_tz = 0, corr_tz = 0.
- * reload: re-read from path (NULL = the default location). */
+ * corr_tz", and a "batch" line, below), read from $TTSH_TUNE_FILE or
+ * tuned.txt beside libttsh3d.so and applied when ttsh_params leaves
+ * rhs_variant = -1, rhs_tz = 0, corr_tz = 0.  reload: re-read from path
+ * (NULL = the default location). */
 int ttsh_tuning_reload(const char* path);
 int ttsh_tuning_lookup(int N, int* rhs_variant, int* rhs_tz, int* corr_tz);
+/* New: the layout of a batch whose droplets fill the device (table line
+ * "batch rhs_variant rhs_tz corr_tz"): the K_rhs tile every droplet then
+ * runs, the minimum K_rhs z-range per work item and the divergence and
+ * corrector z-range.  Without the line the engine uses 32x8, 32 and 32.  A
+ * reload applies to running batches at their next step call. */
+int ttsh_tuning_lookup_batch(int* rhs_variant, int* rhs_tz, int* corr_tz);
 
 #ifdef __cplusplus
 }

@@ -260,3 +260,22 @@ def test_cli_rejects_bad_sentinel(tmp_path):
     r = subprocess.run([exe, "7"], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1
     assert "Error while reading the input file!" in r.stdout
+
+
+def test_tuning_table_batch_line(ttsh, tmp_path):
+    """The tuning table's per-size lines and its device-filling batch line
+    ("batch rhs_variant rhs_tz corr_tz") round-trip through the library."""
+    from ttsh3d import autotune
+    path = str(tmp_path / "tuned.txt")
+    autotune.write_table({64: (4, 8, 4), 128: (5, 16, 16)}, path, (1, 48, 16))
+    assert autotune.read_table(path) == ({64: (4, 8, 4), 128: (5, 16, 16)}, (1, 48, 16))
+    try:
+        ttsh.tuning_reload(path)
+        assert ttsh.tuning_lookup(128) == (5, 16, 16)
+        assert ttsh.tuning_lookup(96) is None
+        assert ttsh.tuning_lookup_batch() == (1, 48, 16)
+        autotune.write_table({64: (4, 8, 4)}, path)
+        ttsh.tuning_reload(path)
+        assert ttsh.tuning_lookup_batch() is None
+    finally:
+        ttsh.tuning_reload(None)
