@@ -62,12 +62,52 @@ __device__ __forceinline__ float2 rot48(float2 a)
     else return b;
 }
 
+// cos(2 pi r/240) for r in [0, 60]; sin(2 pi r/240) = cos(2 pi (60 - r)/240).
+// The first quadrant of the 240th roots of unity, for the register DFTs
+// whose length divides 240 but not 48 (5, 10, 15, 20).
+__device__ constexpr float kCos240[61] = {
+    1.0f, 0.99965732497555728004f, 0.99862953475457387378f, 0.9969173337331279762f,
+    0.99452189536827333692f, 0.99144486137381041114f, 0.98768834059513772619f, 0.98325490756395458455f,
+    0.97814760073380563793f, 0.97236992039767660183f, 0.96592582628906828675f, 0.95881973486819304976f,
+    0.95105651629515357212f, 0.94264149109217839478f, 0.93358042649720174899f, 0.92387953251128675613f,
+    0.9135454576426008955f, 0.90258528434986060676f, 0.89100652418836786236f, 0.87881711266196537413f,
+    0.86602540378443864676f, 0.85264016435409222152f, 0.83867056794542402964f, 0.82412618862201566173f,
+    0.8090169943749474241f, 0.79335334029123516458f, 0.77714596145697087998f, 0.76040596560003093817f,
+    0.74314482547739423501f, 0.72537437101228763799f, 0.7071067811865475244f, 0.68835457569375398439f,
+    0.66913060635885821383f, 0.64944804833018365573f, 0.62932039104983745271f, 0.60876142900872063942f,
+    0.58778525229247312917f, 0.56640623692483283182f, 0.54463903501502708222f, 0.52249856471594886499f,
+    0.5f, 0.47715876025960841505f, 0.45399049973954679156f, 0.43051109680829514438f,
+    0.40673664307580020775f, 0.38268343236508977173f, 0.35836794954530027348f, 0.33380685923377092883f,
+    0.3090169943749474241f, 0.28401534470392261744f, 0.25881904510252076235f, 0.23344536385590541177f,
+    0.2079116908177593371f, 0.1822355254921474566f, 0.15643446504023086901f, 0.13052619222005159155f,
+    0.1045284632676534714f, 0.078459095727844945033f, 0.052335956242943832722f, 0.026176948307873152611f,
+    8.478427660368899644e-32f};
+
+// a * exp(i 2 pi m/240) for a compile-time m (quarter turns are exact swaps).
+template <int M>
+__device__ __forceinline__ float2 rot240(float2 a)
+{
+    constexpr int m = ((M % 240) + 240) % 240;
+    constexpr int q = m / 60, r = m % 60;
+    float2 b = a;
+    if constexpr (r != 0) b = mul2(a, f2(kCos240[r], kCos240[60 - r]));
+    if constexpr (q == 1) return f2(-b.y, b.x);
+    else if constexpr (q == 2) return f2(-b.x, -b.y);
+    else if constexpr (q == 3) return f2(b.y, -b.x);
+    else return b;
+}
+
 // a * W_R^{DIR*e}, W_R = exp(2 pi i/R) with DIR = -1 forward, +1 inverse.
+// Lengths dividing 48 take the 48th-root table, the others the 240th-root one.
 template <int R, int DIR, int E>
 __device__ __forceinline__ float2 twid(float2 a)
 {
-    static_assert(48 % R == 0, "register DFT length must divide 48");
-    return rot48<DIR * E * (48 / R)>(a);
+    if constexpr (48 % R == 0) {
+        return rot48<DIR * E * (48 / R)>(a);
+    } else {
+        static_assert(240 % R == 0, "register DFT length must divide 48 or 240");
+        return rot240<DIR * E * (240 / R)>(a);
+    }
 }
 
 template <int R, int DIR>
@@ -114,6 +154,34 @@ struct Dft<4, DIR> {
         v[2] = sub2(s0, s1);
         v[1] = add2(d0, j);
         v[3] = sub2(d0, j);
+    }
+};
+
+// Length 5: with t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3 and
+// c_k = cos(2 pi k/5), s_k = sin(2 pi k/5):
+//   X0 = x0 + t1 + t2,
+//   X1, X4 = x0 + c1 t1 + c2 t2 -+ DIR i (s1 t3 + s2 t4),
+//   X2, X3 = x0 + c2 t1 + c1 t2 -+ DIR i (s2 t3 - s1 t4)   (DIR = -1 forward).
+template <int DIR>
+struct Dft<5, DIR> {
+    __device__ __forceinline__ static void run(float2* v)
+    {
+        constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        constexpr float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 t1 = add2(v[1], v[4]), t2 = add2(v[2], v[3]);
+        const float2 t3 = sub2(v[1], v[4]), t4 = sub2(v[2], v[3]);
+        const float2 a1 = f2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        const float2 a2 = f2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        const float2 b1 = f2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        const float2 b2 = f2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        // DIR * i * b
+        const float2 j1 = f2(-(float)DIR * b1.y, (float)DIR * b1.x);
+        const float2 j2 = f2(-(float)DIR * b2.y, (float)DIR * b2.x);
+        v[0] = add2(v[0], add2(t1, t2));
+        v[1] = add2(a1, j1);
+        v[4] = sub2(a1, j1);
+        v[2] = add2(a2, j2);
+        v[3] = sub2(a2, j2);
     }
 };
 
@@ -165,6 +233,9 @@ template <int DIR> struct Dft<6, DIR> : DftComp<2, 3, DIR> {};
 template <int DIR> struct Dft<8, DIR> : DftComp<2, 4, DIR> {};
 template <int DIR> struct Dft<12, DIR> : DftComp<4, 3, DIR> {};
 template <int DIR> struct Dft<16, DIR> : DftComp<4, 4, DIR> {};
+template <int DIR> struct Dft<10, DIR> : DftComp<2, 5, DIR> {};
+template <int DIR> struct Dft<15, DIR> : DftComp<3, 5, DIR> {};
+template <int DIR> struct Dft<20, DIR> : DftComp<4, 5, DIR> {};
 
 // Line plan: N = R1*R2, T = max(R1,R2) threads per line, L lines per work-group.
 template <int N>
@@ -172,10 +243,13 @@ struct Plan;
 template <> struct Plan<32>  { static constexpr int R1 = 4,  R2 = 8,  L = 32; };
 template <> struct Plan<48>  { static constexpr int R1 = 6,  R2 = 8,  L = 16; };
 template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  L = 32; };
+template <> struct Plan<80>  { static constexpr int R1 = 10, R2 = 8,  L = 16; };
 template <> struct Plan<96>  { static constexpr int R1 = 8,  R2 = 12, L = 16; };
 template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  L = 16; };
 template <> struct Plan<144> { static constexpr int R1 = 12, R2 = 12, L = 16; };
+template <> struct Plan<160> { static constexpr int R1 = 10, R2 = 16, L = 16; };
 template <> struct Plan<192> { static constexpr int R1 = 16, R2 = 12, L = 16; };
+template <> struct Plan<240> { static constexpr int R1 = 16, R2 = 15, L = 16; };
 template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, L = 16; };
 
 template <int N>
@@ -469,10 +543,9 @@ static hipError_t launch_fft_xy_n(int mode, const Drop* drops, const Work* work,
                                   int grid, hipStream_t st)
 {
     const dim3 blk(XyCfg<N>::NT);
-    if (mode == FFT_FWD) hipLaunchKernelGGL((k_fft_xy<N, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else if (mode == FFT_INV) hipLaunchKernelGGL((k_fft_xy<N, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    if (mode == FFT_FWD) return launch_kernel(k_fft_xy<N, FFT_FWD>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    if (mode == FFT_INV) return launch_kernel(k_fft_xy<N, FFT_INV>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    return hipErrorInvalidValue;
 }
 
 bool fft_xy_fused(int N) { return N <= kMaxFusedXy; }
@@ -485,6 +558,7 @@ hipError_t launch_fft_xy(int N, int mode, const Drop* drops, const Work* work, c
     case 32: return launch_fft_xy_n<32>(mode, drops, work, nwork_p, kstep, grid, st);
     case 48: return launch_fft_xy_n<48>(mode, drops, work, nwork_p, kstep, grid, st);
     case 64: return launch_fft_xy_n<64>(mode, drops, work, nwork_p, kstep, grid, st);
+    case 80: return launch_fft_xy_n<80>(mode, drops, work, nwork_p, kstep, grid, st);
     case 96: return launch_fft_xy_n<96>(mode, drops, work, nwork_p, kstep, grid, st);
     case 128: return launch_fft_xy_n<128>(mode, drops, work, nwork_p, kstep, grid, st);
     default: return hipErrorInvalidValue;
@@ -496,13 +570,13 @@ static hipError_t launch_fft_n(int axis, int mode, const Drop* drops, const Work
                                int kstep, int grid, hipStream_t st)
 {
     const dim3 blk(PlanT<N>::NB);
-    if (axis == 0 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 0, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else if (axis == 0 && mode == FFT_INV) hipLaunchKernelGGL((k_fft<N, 0, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else if (axis == 1 && mode == FFT_FWD) hipLaunchKernelGGL((k_fft<N, 1, FFT_FWD>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else if (axis == 1 && mode == FFT_INV) hipLaunchKernelGGL((k_fft<N, 1, FFT_INV>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else if (axis == 2 && mode == FFT_POISSON) hipLaunchKernelGGL((k_fft<N, 2, FFT_POISSON>), dim3(grid), blk, 0, st, drops, work, nwork_p, kstep);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    if (axis == 0 && mode == FFT_FWD) return launch_kernel(k_fft<N, 0, FFT_FWD>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    if (axis == 0 && mode == FFT_INV) return launch_kernel(k_fft<N, 0, FFT_INV>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    if (axis == 1 && mode == FFT_FWD) return launch_kernel(k_fft<N, 1, FFT_FWD>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    if (axis == 1 && mode == FFT_INV) return launch_kernel(k_fft<N, 1, FFT_INV>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    if (axis == 2 && mode == FFT_POISSON)
+        return launch_kernel(k_fft<N, 2, FFT_POISSON>, dim3(grid), blk, st, drops, work, nwork_p, kstep);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_fft(int N, int axis, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep,
@@ -513,10 +587,13 @@ hipError_t launch_fft(int N, int axis, int mode, const Drop* drops, const Work* 
     case 32: return launch_fft_n<32>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 48: return launch_fft_n<48>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 64: return launch_fft_n<64>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 80: return launch_fft_n<80>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 96: return launch_fft_n<96>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 128: return launch_fft_n<128>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 144: return launch_fft_n<144>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 160: return launch_fft_n<160>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 192: return launch_fft_n<192>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    case 240: return launch_fft_n<240>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     case 256: return launch_fft_n<256>(axis, mode, drops, work, nwork_p, kstep, grid, st);
     default: return hipErrorInvalidValue;
     }
@@ -528,10 +605,13 @@ int fft_lines_per_group(int N)
     case 32: return PlanT<32>::L;
     case 48: return PlanT<48>::L;
     case 64: return PlanT<64>::L;
+    case 80: return PlanT<80>::L;
     case 96: return PlanT<96>::L;
     case 128: return PlanT<128>::L;
     case 144: return PlanT<144>::L;
+    case 160: return PlanT<160>::L;
     case 192: return PlanT<192>::L;
+    case 240: return PlanT<240>::L;
     case 256: return PlanT<256>::L;
     default: return 0;
     }

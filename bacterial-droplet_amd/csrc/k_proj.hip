@@ -563,17 +563,15 @@ hipError_t launch_div(const Drop* drops, const Work* work, const int* nwork_p, i
                       hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_div<kProjTX, kProjTY>), dim3(grid), dim3(kProjTX * kProjTY), 0, st, drops, work, nwork_p,
-                       parity, kstep);
-    return hipGetLastError();
+    return launch_kernel(k_div<kProjTX, kProjTY>, dim3(grid), dim3(kProjTX * kProjTY), st, drops, work, nwork_p,
+                         parity, kstep);
 }
 
 hipError_t launch_proj_stats(const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
                              hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_proj_stats, dim3(grid), dim3(kStatsThreads), 0, st, drops, work, nwork_p, kstep);
-    return hipGetLastError();
+    return launch_kernel(k_proj_stats, dim3(grid), dim3(kStatsThreads), st, drops, work, nwork_p, kstep);
 }
 
 // One thread per (z, y): the two end columns of the six predictor fields.
@@ -622,9 +620,8 @@ hipError_t launch_corr(const Drop* drops, const Work* work, const int* nwork_p, 
                        hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_corr<kProjTX, kProjTY>), dim3(grid), dim3(kProjTX * kProjTY), 0, st, drops, work, nwork_p,
-                       parity, kstep);
-    return hipGetLastError();
+    return launch_kernel(k_corr<kProjTX, kProjTY>, dim3(grid), dim3(kProjTX * kProjTY), st, drops, work, nwork_p,
+                         parity, kstep);
 }
 
 hipError_t launch_reduce(const Drop* drops, int ndrops, int parity, double* part, int nblk, double* out,

@@ -518,8 +518,7 @@ static hipError_t launch_rhs_t(const Drop* drops, const Work* work, const int* n
                                int grid, hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_rhs<TX, TY, MINB>), dim3(grid), dim3(TX * TY), 0, st, drops, work, nwork_p, parity, kstep);
-    return hipGetLastError();
+    return launch_kernel(k_rhs<TX, TY, MINB>, dim3(grid), dim3(TX * TY), st, drops, work, nwork_p, parity, kstep);
 }
 
 // Tile variants: TX x TY cells per work-group and the work-groups per CU the

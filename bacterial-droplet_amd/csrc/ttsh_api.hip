@@ -36,6 +36,8 @@
 
 namespace ttsh {
 
+thread_local LaunchEvents g_launch_ev;
+
 #define HIPCHK(expr)                                                                                  \
     do {                                                                                              \
         hipError_t e_ = (expr);                                                                       \
@@ -789,42 +791,68 @@ struct Engine {
         return m;
     }
 
-    // One step's launches.  graph: use the captured fixed grids.  ev (KK_N+1
-    // events, may be null) is recorded at every kind boundary.
-    int launch_step(int parity, int kstep, bool graph, hipEvent_t* ev = nullptr)
+    // One step's launches.  graph: use the captured fixed grids.  ev (2*KK_N
+    // events, may be null): kind k's first launch records ev[2k] at its start
+    // and its last launch ev[2k+1] at its end, from the dispatch itself
+    // (hipExtLaunchKernel), so the events time the kernels as a kernel trace
+    // does; launched[k] tells whether kind k launched at all.
+    int launch_step(int parity, int kstep, bool graph, hipEvent_t* ev = nullptr, bool* launched = nullptr)
     {
         // plain launches: one work-group per visit of the XCD-blocked order
         auto grid = [&](int t) { return graph ? ggrid[t] : ((tab[t].n + kXcds - 1) / kXcds) * kXcds; };
         auto on = [&](int s) { return graph ? ((gsizes >> s) & 1u) != 0 : tab[t_z(s)].n > 0; };
-        auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
         const int* cnt = d_counts;
-        HIPCHK(mark(KK_RHS));
-        HIPCHK(launch_rhs(rhs_active, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st));
-        HIPCHK(mark(KK_DIV));
-        HIPCHK(launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st));
-        HIPCHK(launch_proj_stats(d_drops, tab[T_STAT].d, cnt + T_STAT, kstep, grid(T_STAT), st));
+        // the launches of the step in order, with their kind
+        struct L {
+            int kind, what, s, t;
+        };
+        enum { W_RHS, W_DIV, W_STAT, W_FFT, W_FFT_XY, W_CORR };
+        L list[4 + 5 * kNumSizes];
+        int nl = 0;
+        auto add = [&](int kind, int what, int s, int t) {
+            if (grid(t) > 0) list[nl++] = L{kind, what, s, t};
+        };
+        add(KK_RHS, W_RHS, 0, T_RHS);
+        add(KK_DIV, W_DIV, 0, T_PROJ);
+        add(KK_DIV, W_STAT, 0, T_STAT);
         // x, y, z-Poisson, y inverse, x inverse; sizes with the fused x+y
         // pass do x+y in the x slots and nothing in the y slots
         const int axes[5] = {0, 1, 2, 1, 0};
         const int modes[5] = {FFT_MODE_FWD, FFT_MODE_FWD, FFT_MODE_POISSON, FFT_MODE_INV, FFT_MODE_INV};
-        for (int f = 0; f < 5; ++f) {
-            HIPCHK(mark(KK_FX + f));
+        for (int f = 0; f < 5; ++f)
             for (int s = 0; s < kNumSizes; ++s) {
                 if (!on(s)) continue;
-                const int N = kSizes[s];
                 const bool fused = (((graph ? gxy_mask : xy_mask) >> s) & 1u) != 0;
                 if (axes[f] != 2 && fused) {
-                    if (axes[f] == 0)
-                        HIPCHK(launch_fft_xy(N, modes[f], d_drops, tab[t_pl(s)].d, cnt + t_pl(s), kstep, grid(t_pl(s)), st));
+                    if (axes[f] == 0) add(KK_FX + f, W_FFT_XY, s, t_pl(s));
                     continue;
                 }
-                const int t = axes[f] == 2 ? t_z(s) : t_xy(s);
-                HIPCHK(launch_fft(N, axes[f], modes[f], d_drops, tab[t].d, cnt + t, kstep, grid(t), st));
+                add(KK_FX + f, W_FFT, s, axes[f] == 2 ? t_z(s) : t_xy(s));
             }
+        add(KK_CORR, W_CORR, 0, T_CORR);
+        if (launched)
+            for (int k = 0; k < KK_N; ++k) launched[k] = false;
+        for (int i = 0; i < nl; ++i) {
+            const L& l = list[i];
+            if (ev) {
+                g_launch_ev.start = (i == 0 || list[i - 1].kind != l.kind) ? ev[2 * l.kind] : nullptr;
+                g_launch_ev.stop = (i == nl - 1 || list[i + 1].kind != l.kind) ? ev[2 * l.kind + 1] : nullptr;
+                if (launched) launched[l.kind] = true;
+            }
+            const int N = kSizes[l.s];
+            const int f = l.kind - KK_FX;
+            hipError_t e = hipSuccess;
+            switch (l.what) {
+            case W_RHS: e = launch_rhs(rhs_active, d_drops, tab[T_RHS].d, cnt + T_RHS, parity, kstep, grid(T_RHS), st); break;
+            case W_DIV: e = launch_div(d_drops, tab[T_PROJ].d, cnt + T_PROJ, parity, kstep, grid(T_PROJ), st); break;
+            case W_STAT: e = launch_proj_stats(d_drops, tab[T_STAT].d, cnt + T_STAT, kstep, grid(T_STAT), st); break;
+            case W_FFT_XY: e = launch_fft_xy(N, modes[f], d_drops, tab[l.t].d, cnt + l.t, kstep, grid(l.t), st); break;
+            case W_FFT: e = launch_fft(N, axes[f], modes[f], d_drops, tab[l.t].d, cnt + l.t, kstep, grid(l.t), st); break;
+            default: e = launch_corr(d_drops, tab[T_CORR].d, cnt + T_CORR, parity, kstep, grid(T_CORR), st); break;
+            }
+            g_launch_ev = LaunchEvents{};
+            HIPCHK(e);
         }
-        HIPCHK(mark(KK_CORR));
-        HIPCHK(launch_corr(d_drops, tab[T_CORR].d, cnt + T_CORR, parity, kstep, grid(T_CORR), st));
-        HIPCHK(mark(KK_N));
         return TTSH_OK;
     }
 
@@ -950,23 +978,28 @@ struct Engine {
         refresh_tuning();
         for (int k = 0; k < KK_N; ++k) ms[k] = 0.0;
         constexpr int64_t kChunk = 256;
-        constexpr int E = KK_N + 1;
+        constexpr int E = 2 * KK_N;
         std::vector<hipEvent_t> ev(E * kChunk);
+        std::vector<char> went(KK_N * kChunk);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         int rc = TTSH_OK;
         while (n > 0 && rc == TTSH_OK) {
             const int64_t m = std::min(n, kChunk);
             normalise_parity();
             if (needs_rebuild(m)) rc = rebuild((int)m);
-            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
-                rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, false, &ev[E * s]);
+            bool launched[KK_N];
+            for (int64_t s = 0; s < m && rc == TTSH_OK; ++s) {
+                rc = launch_step(engine_parity ^ (int)(s & 1), (int)s, false, &ev[E * s], launched);
+                for (int k = 0; k < KK_N; ++k) went[KK_N * s + k] = launched[k];
+            }
             if (rc) break;
-            hipError_t e = hipEventSynchronize(ev[E * (m - 1) + KK_N]);
+            hipError_t e = hipStreamSynchronize(st);
             if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
             for (int64_t s = 0; s < m && rc == TTSH_OK; ++s)
                 for (int k = 0; k < KK_N; ++k) {
+                    if (!went[KK_N * s + k]) continue;
                     float t = 0.f;
-                    e = hipEventElapsedTime(&t, ev[E * s + k], ev[E * s + k + 1]);
+                    e = hipEventElapsedTime(&t, ev[E * s + 2 * k], ev[E * s + 2 * k + 1]);
                     if (e != hipSuccess) { rc = fail(TTSH_EHIP, hipGetErrorString(e)); break; }
                     ms[k] += t;
                 }

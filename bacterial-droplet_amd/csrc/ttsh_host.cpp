@@ -38,29 +38,23 @@ int fft_plan(int N, int* rad, int maxst)
     while (m % 4 == 0 && n < maxst) { rad[n++] = 4; m /= 4; }
     while (m % 2 == 0 && n < maxst) { rad[n++] = 2; m /= 2; }
     while (m % 3 == 0 && n < maxst) { rad[n++] = 3; m /= 3; }
+    while (m % 5 == 0 && n < maxst) { rad[n++] = 5; m /= 5; }
     return (m == 1) ? n : 0;
 }
 
+// The grid sizes with a compile-time FFT plan (k_fft.hip).
+static const int kPlanSizes[] = {32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256};
+
 bool supported_N(int N, std::string* why)
 {
-    int rad[16];
-    if (N < 32 || N > 256 || N % 16 != 0) {
-        if (why) *why = "N must be a multiple of 16 in [32, 256], got " + std::to_string(N);
-        return false;
+    for (int s : kPlanSizes)
+        if (s == N) return true;
+    if (why) {
+        std::string list;
+        for (int s : kPlanSizes) list += (list.empty() ? "" : ", ") + std::to_string(s);
+        *why = "no FFT plan for N = " + std::to_string(N) + " (sizes: " + list + ")";
     }
-    const int nst = fft_plan(N, rad, 16);
-    if (nst == 0 || nst > 8) {
-        if (why) *why = "N must have no prime factor other than 2 and 3, got " + std::to_string(N);
-        return false;
-    }
-    int rmin = 4;
-    for (int i = 0; i < nst; ++i) rmin = rad[i] < rmin ? rad[i] : rmin;
-    // butterflies per stage of the 16-line Y/Z passes must fit 4 per thread of 256
-    if (16 * (N / rmin) > 4 * 256) {
-        if (why) *why = "N = " + std::to_string(N) + " exceeds the FFT work-group capacity";
-        return false;
-    }
-    return true;
+    return false;
 }
 
 // ---------------------------------------------------------------------------

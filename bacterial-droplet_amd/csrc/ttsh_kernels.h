@@ -4,9 +4,31 @@
 // a grid size; the kernels loop over work items with a grid stride and skip
 // items whose droplet has no step left at index kstep.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "ttsh_dev.h"
 
 namespace ttsh {
+
+// Events that the next step-kernel launch on this thread records from its
+// own dispatch (start) and completion (stop), as a profiler's kernel trace
+// times it, with no marker of its own between kernels.  A launch clears them.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchEvents g_launch_ev;
+
+template <typename K, typename... Args>
+inline hipError_t launch_kernel(K kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
+{
+    if (g_launch_ev.start || g_launch_ev.stop) {
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, g_launch_ev.start, g_launch_ev.stop, 0, args...);
+        g_launch_ev = LaunchEvents{};
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    }
+    return hipGetLastError();
+}
 
 // K_rhs tile variants (TX x TY cells per work-group, one cell per thread, and
 // the occupancy the register allocation targets); see kRhsVars in k_rhs.hip.
@@ -17,9 +39,9 @@ enum RhsVariant { RHS_32x8 = 0, RHS_16x16 = 1, RHS_64x4 = 2, RHS_16x8 = 3, RHS_3
 constexpr int kProjTX = 32;
 constexpr int kProjTY = 8;
 
-// Grid sizes with a compile-time FFT plan.
-constexpr int kNumSizes = 8;
-constexpr int kSizes[kNumSizes] = {32, 48, 64, 96, 128, 144, 192, 256};
+// Grid sizes with a compile-time FFT plan (prime factors 2, 3 and 5).
+constexpr int kNumSizes = 11;
+constexpr int kSizes[kNumSizes] = {32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256};
 
 // FFT pass kinds (k_fft.hip).
 enum { FFT_MODE_FWD = 0, FFT_MODE_INV = 1, FFT_MODE_POISSON = 2 };

@@ -10,11 +10,12 @@ cA = -2, random-unit P) is `value`; the 256^3 droplet (R = 80, xi = 3, noisy
 aster P), whose 1.55 GB working set exceeds the 256 MB Infinity Cache, is
 reported beside it.  A "step" is one explicit-Euler step of the reference's
 EEuler (TTSH3D.cu:347-359) on the droplet, device-resident.  W warm-up
-steps, then K steps with HIP events recorded on the step's stream between
-consecutive kernel kinds (kernel_ms: each kind's average duration; these
-add launch gaps), then exactly K timed steps between two synchronisations,
-launched as a caller launches them (ttsh_step: the step's kernels queued back
-to back on one stream); value = N^3 * K / wall seconds / 1e6.
+steps, then K steps through ttsh_profile, whose kernels record HIP events at
+their own start and end on the step's stream (kernel_ms: each kind's average
+duration, as a kernel trace measures it), then exactly K timed steps between
+two synchronisations, launched as a caller launches them (ttsh_step: the
+step's kernels queued back to back on one stream); value = N^3 * K / wall
+seconds / 1e6.
 
 roofline: the dominant kernel's algorithmic bytes per launch (FP32 field
 traffic: K_rhs and K_corr 56 B/cell, K_div 32, each FFT pass 16) divided by
@@ -114,8 +115,8 @@ def single_leg(ttsh, N, steps, warmup):
     sim.set_state(phi, P, v)
     sim.step(warmup)
     sim.synchronize()
-    # per-kernel durations: K steps with HIP events recorded on the step's
-    # stream at every kernel-kind boundary (plain launches)
+    # per-kernel durations: K steps whose kernels record HIP events at their
+    # own start and end on the step's stream (plain launches)
     t1 = time.perf_counter()
     kms = sim.profile(steps)
     sim.synchronize()

@@ -17,12 +17,12 @@
  *   - A ttsh_sim or ttsh_batch owns its device memory and one HIP stream.
  *     Handles are not thread-safe; use one handle per thread.
  *   - Grids are cubic (Nx = Ny = Nz = N, as the reference's cuFFT plan
- *     requires, TTSH3D.cu:1123) with N in {32, 48, 64, 96, 128, 144, 192,
- *     256}: the sizes with a compile-time FFT plan (prime factors 2 and 3).
- *     The reference's default and swept N is 64 (run_cuda.pl:14-16,
- *     forEach.pl:30); any other N its cuFFT plan accepts -- below 32, above
- *     256, or with a prime factor of 5 or more (e.g. 16, 40, 80, 100, 160)
- *     -- is refused here with TTSH_EUNSUPPORTED.
+ *     requires, TTSH3D.cu:1123) with N in {32, 48, 64, 80, 96, 128, 144,
+ *     160, 192, 240, 256}: the sizes with a compile-time FFT plan (prime
+ *     factors 2, 3 and 5).  The reference's default and swept N is 64
+ *     (run_cuda.pl:14-16, forEach.pl:30); any other N its cuFFT plan accepts
+ *     -- below 32, above 256, or not in that list (e.g. 16, 40, 100, 200) --
+ *     is refused here with TTSH_EUNSUPPORTED.
  */
 #ifndef TTSH3D_H
 #define TTSH3D_H
@@ -146,11 +146,13 @@ int ttsh_write_conf_arrays(const char* path, int N, int ExpoData, const float* p
 int ttsh_run(ttsh_sim* s, const char* run_dir, int64_t* steps_done);
 
 /* New (the reference only prints host CPU time, TTSH3D.cu:266,284-286): runs n
- * steps like ttsh_step with a HIP event recorded between consecutive kernels
- * on the handle's stream; kernel_ms[k] receives the summed milliseconds of
- * kernel kind k (8 kinds, names from ttsh_kernel_name: K_rhs, K_div (with
- * the per-droplet reduction of its sums that follows it), the five FFT
- * passes, K_corr).  Synchronises once per 256 steps. */
+ * steps like ttsh_step, each kernel kind's first launch recording a HIP
+ * event at its start and its last launch one at its end, from the kernel
+ * dispatch itself (hipExtLaunchKernel: no marker between kernels), on the
+ * handle's stream; kernel_ms[k] receives the summed milliseconds of kernel
+ * kind k (8 kinds, names from ttsh_kernel_name: K_rhs, K_div (with the
+ * per-droplet reduction of its sums that follows it), the five FFT passes,
+ * K_corr).  Synchronises once per 256 steps. */
 int ttsh_profile(ttsh_sim* s, int64_t n, double kernel_ms[8]);
 
 /* Name of kernel kind k of ttsh_profile ("" outside 0..7). */

@@ -105,7 +105,8 @@ def test_short_or_missing_input(ttsh, tmp_path):
     (dict(TimeScheme=3), "EUNSUPPORTED"),
     (dict(Nx=64, Ny=64, Nz=32), "EUNSUPPORTED"),
     (dict(N=100), "EUNSUPPORTED"),
-    (dict(N=80), "EUNSUPPORTED"),
+    (dict(N=40), "EUNSUPPORTED"),
+    (dict(N=16), "EUNSUPPORTED"),
     (dict(N=512), "EUNSUPPORTED"),
     (dict(Nb=0), "EINVAL"),
     (dict(dt0=0.0), "EINVAL"),
@@ -279,3 +280,10 @@ def test_tuning_table_batch_line(ttsh, tmp_path):
         assert ttsh.tuning_lookup_batch() is None
     finally:
         ttsh.tuning_reload(None)
+
+
+@pytest.mark.parametrize("N", [32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256])
+def test_every_planned_size_validates(ttsh, N):
+    """The sizes with a compile-time FFT plan (prime factors 2, 3 and 5)."""
+    p = ttsh.default_params(N=N)
+    assert ttsh.lib().ttsh_validate_params(ctypes.byref(p)) == 0

@@ -3,7 +3,7 @@
 getTempVari_2 and getDtF (TTSH3D.cu:561-632) couple the fields through
 advection, the Cahn-Hilliard chemical potential, the capillary force, the
 active stress, the polarization's molecular field, the lambda_0 self-advection
-and the anchoring term.  Each test below builds a state on which every term
+and the anchoring term.  Each case below builds a state on which every term
 of dt v, dt P and dt phi reduces to a pointwise closed form, using only the
 Fourier symbols of the reference's stencils (pinned on their own in
 test_oracle_kat.py::test_stencil_symbols):
@@ -14,9 +14,11 @@ test_oracle_kat.py::test_stencil_symbols):
 
 Products of modes are expanded into sum and difference wavevectors, and powers
 of cos into harmonics, so no expected value uses a stencil or an oracle
-output.  The check functions take the oracle module as an argument, so
-tests/test_oracle_mutations.py can run them against copies of the oracle with
-a planted mistake and show that each such mistake fails one of them.
+output.  The builders take the grid size, so tests/test_gpu_coupling.py
+checks the same cases on the HIP step; the check functions take the oracle
+module as an argument, so tests/test_oracle_mutations.py can run them against
+copies of the oracle with a planted mistake and show that each such mistake
+fails one of them.
 """
 import numpy as np
 import pytest
@@ -26,15 +28,15 @@ H = 1.1
 DT = 0.01
 
 
-def grid(n=N):
+def grid(n):
     return np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")  # z, y, x
 
 
-def kvec(m, n=N, h=H):
+def kvec(m, n, h=H):
     return np.array([2 * np.pi * mi / (n * h) for mi in m], dtype=np.float64)
 
 
-def theta(k, phase, n=N, h=H):
+def theta(k, phase, n, h=H):
     z, y, x = grid(n)
     return k[0] * x * h + k[1] * y * h + k[2] * z * h + phase
 
@@ -57,25 +59,12 @@ def lsym(k, h=H):
 
 
 def vec(a, f):
-    """(3,) amplitude times a scalar field -> (3, N, N, N)."""
+    """(3,) amplitude times a scalar field -> (3, n, n, n)."""
     return np.asarray(a, dtype=np.float64)[:, None, None, None] * f[None]
 
 
-def const_vec(a, n=N):
+def const_vec(a, n):
     return np.asarray(a, dtype=np.float64)[:, None, None, None] * np.ones((3, n, n, n))
-
-
-def one_step(O, phi, P, v, **prm):
-    o = O.Oracle(N, faithful=False, h=H, dt0=DT, **prm)
-    o.set_state(phi, P, v)
-    o.step(1)
-    return o
-
-
-def close(got, exp, what, rtol=1e-12):
-    scale = max(1.0, float(np.abs(exp).max()))
-    err = float(np.abs(np.asarray(got) - exp).max())
-    assert err <= rtol * scale, f"{what}: max error {err:.3e} (scale {scale:.3e})"
 
 
 def chemical_potential(phi0, eps, k, th, aPhi, kPhi):
@@ -98,17 +87,23 @@ def chemical_potential(phi0, eps, k, th, aPhi, kPhi):
     return mu0, lap_mu0
 
 
+def case(phi, P, v, prm, dtphi, dtv, dtP, **extra):
+    """A state, its parameters, the exact dt phi, dt v, dt P, and optional
+    exact intermediates (mu0, mu, P2) that only the oracle exposes."""
+    return dict(phi=phi, P=P, v=v, params=prm, dtphi=dtphi, dtv=dtv, dtP=dtP, extra=extra)
+
+
 # ---------------------------------------------------------------------------
-# The checks (each takes the oracle module)
+# The cases, on an n^3 grid
 # ---------------------------------------------------------------------------
 
-def check_advection(O):
+def build_advection(n):
     """Transport by the flow, TTSH3D.cu:610-616,631: with P = 0 and
     aPhi = kPhi = 0 (so mu0 = 0), dt phi = -v.D phi and
     dt v_b = -v.D v_b + mu lap v_b for v = v0 + a cos(th2) (a not transverse:
     the self-advection (a.d) a_b sin cos term is included), phi = phi0 + e cos(th1)."""
-    k1, k2 = kvec((3, 1, 2)), kvec((1, 2, 5))
-    th1, th2 = theta(k1, 0.4), theta(k2, -0.9)
+    k1, k2 = kvec((3, 1, 2), n), kvec((1, 2, 5), n)
+    th1, th2 = theta(k1, 0.4, n), theta(k2, -0.9, n)
     d1, d2 = dsym(k1), dsym(k2)
     l2 = lsym(k2)
     phi0, eps = 0.6, 0.2
@@ -116,48 +111,39 @@ def check_advection(O):
     a = np.array([0.15, 0.25, -0.1])
     mu = 0.7
     phi = phi0 + eps * np.cos(th1)
-    v = const_vec(v0) + vec(a, np.cos(th2))
-    P = np.zeros((3, N, N, N))
-    o = one_step(O, phi, P, v, aPhi=0.0, kPhi=0.0, mu=mu, cA=-1.3, alpha=-0.02, beta=4.366,
-                 eta=-0.187, lambda0=9.0, A=0.5)
-    exp_dtphi = eps * np.sin(th1) * ((v0 @ d1) + (a @ d1) * np.cos(th2))
-    close(o.aux("dtphi"), exp_dtphi, "dt phi (advection)")
+    v = const_vec(v0, n) + vec(a, np.cos(th2))
+    P = np.zeros((3, n, n, n))
+    prm = dict(aPhi=0.0, kPhi=0.0, mu=mu, cA=-1.3, alpha=-0.02, beta=4.366, eta=-0.187, lambda0=9.0, A=0.5)
+    dtphi = eps * np.sin(th1) * ((v0 @ d1) + (a @ d1) * np.cos(th2))
     adv = np.sin(th2) * ((v0 @ d2) + (a @ d2) * np.cos(th2))
-    exp_dtv = vec(a, adv + mu * l2 * np.cos(th2))
-    close(o.aux("dtv"), exp_dtv, "dt v (advection + viscosity)")
-    close(o.aux("dtP"), np.zeros((3, N, N, N)), "dt P with P = 0")
-    close(o.aux("mu0"), np.zeros((N, N, N)), "mu0 with aPhi = kPhi = 0")
+    dtv = vec(a, adv + mu * l2 * np.cos(th2))
+    return case(phi, P, v, prm, dtphi, dtv, np.zeros((3, n, n, n)), mu0=np.zeros((n, n, n)))
 
 
-def check_cahn_hilliard_capillary(O):
+def build_cahn_hilliard_capillary(n):
     """Chemical potential and capillary force, TTSH3D.cu:565,610-616,631:
     v = P = 0, phi = phi0 + e cos(th): mu0 and gamma*lap(mu0) through the
     harmonics cos(n th), n <= 3, and dt v_b = mu0 * D_b phi."""
-    k = kvec((2, 3, 1))
-    th = theta(k, 0.25)
+    k = kvec((2, 3, 1), n)
+    th = theta(k, 0.25, n)
     d = dsym(k)
     phi0, eps = 0.45, 0.3
     aPhi, kPhi, gamma = 0.8, 1.3, 0.9
     phi = phi0 + eps * np.cos(th)
-    zero = np.zeros((3, N, N, N))
-    o = one_step(O, phi, zero, zero, aPhi=aPhi, kPhi=kPhi, gamma=gamma, mu=1.1, cA=-2.0, A=0.7)
+    zero = np.zeros((3, n, n, n))
+    prm = dict(aPhi=aPhi, kPhi=kPhi, gamma=gamma, mu=1.1, cA=-2.0, A=0.7)
     mu0, lap_mu0 = chemical_potential(phi0, eps, k, th, aPhi, kPhi)
-    close(o.aux("mu0"), mu0, "mu0")
-    close(o.aux("dtphi"), gamma * lap_mu0, "dt phi (Cahn-Hilliard)")
-    close(o.aux("dtv"), vec(-eps * d, mu0 * np.sin(th)), "dt v (capillary mu0 D phi)")
-    close(o.aux("dtP"), zero, "dt P with P = 0")
-    ophi, _, _ = o.get_state()
-    close(ophi, phi + DT * gamma * lap_mu0, "phi^{n+1} = phi + Dt dt phi")
+    return case(phi, zero, zero, prm, gamma * lap_mu0, vec(-eps * d, mu0 * np.sin(th)), zero, mu0=mu0)
 
 
-def check_active_stress(O):
+def build_active_stress(n):
     """Active stress and molecular field, TTSH3D.cu:573-575,587-595,610-630:
     v = 0, aPhi = kPhi = 0, lambda0 = A = 0, phi = phi0 + e cos(th2) and a
     transverse polarization mode P = a cos(th1) (a.d1 = 0).
     dt v_b = cA sum_a D_a(phi D_a P_b) expands into the modes k1, k1 +- k2;
     dt P_b = -alpha phi P_b - beta |P|^2 P_b + eta lap(phi P_b + La^2 lap P_b)."""
-    k1, k2 = kvec((2, 1, 3)), kvec((1, 3, 1))
-    th1, th2 = theta(k1, 0.3), theta(k2, 1.1)
+    k1, k2 = kvec((2, 1, 3), n), kvec((1, 3, 1), n)
+    th1, th2 = theta(k1, 0.3, n), theta(k2, 1.1, n)
     d1, dp, dm = dsym(k1), dsym(k1 + k2), dsym(k1 - k2)
     l1, lp, lm = lsym(k1), lsym(k1 + k2), lsym(k1 - k2)
     # a transverse to d1
@@ -168,28 +154,26 @@ def check_active_stress(O):
     cA, alpha, beta, eta, La = -1.7, -0.05, 2.5, -0.3, 1.2
     phi = phi0 + eps * np.cos(th2)
     P = vec(a, np.cos(th1))
-    v = np.zeros((3, N, N, N))
-    o = one_step(O, phi, P, v, aPhi=0.0, kPhi=0.0, cA=cA, alpha=alpha, beta=beta, eta=eta, La=La,
-                 lambda0=0.0, A=0.0, mu=0.9)
+    v = np.zeros((3, n, n, n))
+    prm = dict(aPhi=0.0, kPhi=0.0, cA=cA, alpha=alpha, beta=beta, eta=eta, La=La, lambda0=0.0, A=0.0, mu=0.9)
     tp, tm = th1 + th2, th1 - th2
     # sum_a d1_a [phi0 d1_a cos th1 + e/2 (dp_a cos tp + dm_a cos tm)]
     s = phi0 * (d1 @ d1) * np.cos(th1) + eps / 2 * ((d1 @ dp) * np.cos(tp) + (d1 @ dm) * np.cos(tm))
-    close(o.aux("dtv"), vec(-cA * a, s), "dt v (active stress)")
     c1 = np.cos(th1)
     molecular = phi0 * l1 * c1 + eps / 2 * (lp * np.cos(tp) + lm * np.cos(tm)) + La ** 2 * l1 * l1 * c1
-    exp_dtP = vec(a, -alpha * phi * c1 - beta * (a @ a) * c1 ** 3 + eta * molecular)
-    close(o.aux("dtP"), exp_dtP, "dt P (bulk + molecular field)")
-    close(o.aux("mu"), vec(a, phi * c1 + La ** 2 * l1 * c1), "mu_P = phi P + La^2 lap P")
+    dtP = vec(a, -alpha * phi * c1 - beta * (a @ a) * c1 ** 3 + eta * molecular)
+    return case(phi, P, v, prm, np.zeros((n, n, n)), vec(-cA * a, s), dtP,
+                mu=vec(a, phi * c1 + La ** 2 * l1 * c1))
 
 
-def check_self_advection(O):
+def build_self_advection(n):
     """lambda0 term and active stress with uniform phi, TTSH3D.cu:618-630:
     v = 0, phi = phi0, P = P0 + a cos(th) (a not transverse).
     dt P_b = -alpha phi0 P_b - beta |P|^2 P_b + eta(phi0 lam + La^2 lam^2) a_b cos
              - lambda0 (P.D) P_b,  with -(P.D)P_b = a_b sin (P0.d + (a.d) cos);
     dt v_b = -cA phi0 |d|^2 a_b cos;  dt phi = 0; anchoring vanishes (D phi = 0)."""
-    k = kvec((1, 4, 2))
-    th = theta(k, -0.6)
+    k = kvec((1, 4, 2), n)
+    th = theta(k, -0.6, n)
     d = dsym(k)
     lam = lsym(k)
     phi0 = 0.85
@@ -197,30 +181,26 @@ def check_self_advection(O):
     a = np.array([0.2, 0.1, -0.25])
     cA, alpha, beta, eta, La, lambda0 = 1.4, -0.03, 1.9, -0.25, 0.9, 6.5
     c, sn = np.cos(th), np.sin(th)
-    phi = np.full((N, N, N), phi0)
-    P = const_vec(P0) + vec(a, c)
-    v = np.zeros((3, N, N, N))
-    o = one_step(O, phi, P, v, cA=cA, alpha=alpha, beta=beta, eta=eta, La=La, lambda0=lambda0,
-                 A=0.8, aPhi=1.0, kPhi=1.0)
+    phi = np.full((n, n, n), phi0)
+    P = const_vec(P0, n) + vec(a, c)
+    v = np.zeros((3, n, n, n))
+    prm = dict(cA=cA, alpha=alpha, beta=beta, eta=eta, La=La, lambda0=lambda0, A=0.8, aPhi=1.0, kPhi=1.0)
     P2 = sum((P0[b] + a[b] * c) ** 2 for b in range(3))
-    exp_dtP = (-alpha * phi0 * P - beta * P2[None] * P
-               + vec(a, eta * (phi0 * lam + La ** 2 * lam * lam) * c)
-               + lambda0 * vec(a, sn * ((P0 @ d) + (a @ d) * c)))
-    close(o.aux("dtP"), exp_dtP, "dt P (lambda0 self-advection)")
-    close(o.aux("dtv"), vec(-cA * phi0 * (d @ d) * a, c), "dt v (active stress, uniform phi)")
-    close(o.aux("dtphi"), np.zeros((N, N, N)), "dt phi with uniform phi and v = 0")
-    close(o.aux("P2"), P2, "|P|^2")
+    dtP = (-alpha * phi0 * P - beta * P2[None] * P
+           + vec(a, eta * (phi0 * lam + La ** 2 * lam * lam) * c)
+           + lambda0 * vec(a, sn * ((P0 @ d) + (a @ d) * c)))
+    return case(phi, P, v, prm, np.zeros((n, n, n)), vec(-cA * phi0 * (d @ d) * a, c), dtP, P2=P2)
 
 
-def check_anchoring(O):
+def build_anchoring(n):
     """Anchoring, TTSH3D.cu:597-602,624-630: v = 0, P = P0 uniform,
     phi = phi0 + e cos(th).  D phi = -e d sin, |D phi|^2 = e^2 sin^2 |d|^2, and
     A_bc = D_b phi D_c phi (the reference's 1/3 is an integer zero), so
       dt P_b = -alpha phi P0_b - beta |P0|^2 P0_b + eta e lam cos P0_b
                - A e^2 sin^2 (|d|^2 |P0|^2 P0_b + |P0| d_b (d.P0));
-    dt v_b = mu0 D_b phi and dt phi = gamma lap mu0 (as in the Cahn-Hilliard check)."""
-    k = kvec((3, 2, 1))
-    th = theta(k, 0.8)
+    dt v_b = mu0 D_b phi and dt phi = gamma lap mu0 (as in the Cahn-Hilliard case)."""
+    k = kvec((3, 2, 1), n)
+    th = theta(k, 0.8, n)
     d = dsym(k)
     lam = lsym(k)
     phi0, eps = 0.55, 0.35
@@ -229,66 +209,85 @@ def check_anchoring(O):
     aPhi, kPhi, gamma = 1.2, 0.9, 1.1
     c, sn = np.cos(th), np.sin(th)
     phi = phi0 + eps * c
-    P = const_vec(P0)
-    v = np.zeros((3, N, N, N))
-    o = one_step(O, phi, P, v, alpha=alpha, beta=beta, eta=eta, La=La, A=A, lambda0=lambda0,
-                 aPhi=aPhi, kPhi=kPhi, gamma=gamma, cA=-1.0, mu=1.0)
+    P = const_vec(P0, n)
+    v = np.zeros((3, n, n, n))
+    prm = dict(alpha=alpha, beta=beta, eta=eta, La=La, A=A, lambda0=lambda0, aPhi=aPhi, kPhi=kPhi, gamma=gamma,
+               cA=-1.0, mu=1.0)
     nP2 = P0 @ P0
     anch = A * eps ** 2 * sn ** 2
-    exp_dtP = (-alpha * phi[None] * P - beta * nP2 * P + vec(P0, eta * eps * lam * c)
-               - vec((d @ d) * nP2 * P0 + np.sqrt(nP2) * (d @ P0) * d, anch))
-    close(o.aux("dtP"), exp_dtP, "dt P (anchoring)")
+    dtP = (-alpha * phi[None] * P - beta * nP2 * P + vec(P0, eta * eps * lam * c)
+           - vec((d @ d) * nP2 * P0 + np.sqrt(nP2) * (d @ P0) * d, anch))
     mu0, lap_mu0 = chemical_potential(phi0, eps, k, th, aPhi, kPhi)
-    close(o.aux("dtv"), vec(-eps * d, mu0 * sn), "dt v (capillary)")
-    close(o.aux("dtphi"), gamma * lap_mu0, "dt phi (Cahn-Hilliard)")
-    _, oP, _ = o.get_state()
-    # P^{n+1} = P* projected: P* - Dt D p_p.  P* = P + Dt dtP; dtP has no
-    # divergence-free split in closed form here, so check only the predictor.
-    close(o.aux("pstar"), P + DT * exp_dtP, "P* = P + Dt dt P")
-    assert np.isfinite(oP).all()
+    return case(phi, P, v, prm, gamma * lap_mu0, vec(-eps * d, mu0 * sn), dtP)
 
 
-def check_vorticity_coupling_is_zero(O):
+def build_vorticity_coupling_is_zero(n):
     """The co-rotation and flow-alignment terms OP, DP are integer zeros
     (1/2 * (...), TTSH3D.cu:579-585): a velocity mode with vorticity leaves
     dt P at its value without flow, apart from advection, which is zero for
     uniform P.  v = a cos(th) (transverse), P = P0, phi = phi0."""
-    k = kvec((2, 2, 3))
-    th = theta(k, 0.1)
+    k = kvec((2, 2, 3), n)
+    th = theta(k, 0.1, n)
     d = dsym(k)
     t = np.cross(d, np.array([1.0, 0.2, -0.5]))
     a = 0.4 * t / np.linalg.norm(t)
     phi0 = 0.9
     P0 = np.array([0.3, 0.6, -0.2])
-    alpha, beta = -0.02, 4.366
-    phi = np.full((N, N, N), phi0)
-    P = const_vec(P0)
+    alpha, beta, mu = -0.02, 4.366, 1.0
+    phi = np.full((n, n, n), phi0)
+    P = const_vec(P0, n)
     v = vec(a, np.cos(th))
-    o = one_step(O, phi, P, v, alpha=alpha, beta=beta, kappa=-0.364, lambda0=9.0, A=0.0)
-    exp_dtP = const_vec(-alpha * phi0 * P0 - beta * (P0 @ P0) * P0)
-    close(o.aux("dtP"), exp_dtP, "dt P under a vortical flow")
+    prm = dict(alpha=alpha, beta=beta, kappa=-0.364, lambda0=9.0, A=0.0, mu=mu)
+    dtP = const_vec(-alpha * phi0 * P0 - beta * (P0 @ P0) * P0, n)
+    # v is transverse: -v.D v = (a.d) a sin cos = 0; mu lap v; no phi gradient
+    dtv = vec(a, mu * lsym(k) * np.cos(th))
+    return case(phi, P, v, prm, np.zeros((n, n, n)), dtv, dtP)
 
 
-def check_predictor(O):
-    """getVStar (TTSH3D.cu:471-491): v* = v + Dt dt v and P* = P + Dt dt P,
-    on the advection state, where dt v is known in closed form."""
-    k2 = kvec((1, 2, 5))
-    th2 = theta(k2, -0.9)
-    d2, l2 = dsym(k2), lsym(k2)
-    v0 = np.array([0.3, -0.2, 0.45])
-    a = np.array([0.15, 0.25, -0.1])
-    mu = 0.7
-    phi = np.full((N, N, N), 0.5)
-    v = const_vec(v0) + vec(a, np.cos(th2))
-    P = np.zeros((3, N, N, N))
-    o = one_step(O, phi, P, v, aPhi=0.0, kPhi=0.0, mu=mu)
-    dtv = vec(a, np.sin(th2) * ((v0 @ d2) + (a @ d2) * np.cos(th2)) + mu * l2 * np.cos(th2))
-    close(o.aux("vstar"), v + DT * dtv, "v* = v + Dt dt v")
-    close(o.aux("pstar"), P, "P* = P when P = 0")
+BUILDERS = [build_advection, build_cahn_hilliard_capillary, build_active_stress, build_self_advection,
+            build_anchoring, build_vorticity_coupling_is_zero]
 
 
-CHECKS = [check_advection, check_cahn_hilliard_capillary, check_active_stress, check_self_advection,
-          check_anchoring, check_vorticity_coupling_is_zero, check_predictor]
+# ---------------------------------------------------------------------------
+# Checks against the oracle (each takes the oracle module)
+# ---------------------------------------------------------------------------
+
+def one_step(O, c, n=N):
+    o = O.Oracle(n, faithful=False, h=H, dt0=DT, **c["params"])
+    o.set_state(c["phi"], c["P"], c["v"])
+    o.step(1)
+    return o
+
+
+def close(got, exp, what, rtol=1e-12):
+    scale = max(1.0, float(np.abs(exp).max()))
+    err = float(np.abs(np.asarray(got) - exp).max())
+    assert err <= rtol * scale, f"{what}: max error {err:.3e} (scale {scale:.3e})"
+
+
+def check_case(O, build):
+    c = build(N)
+    o = one_step(O, c)
+    name = build.__name__[len("build_"):]
+    close(o.aux("dtphi"), c["dtphi"], f"{name}: dt phi")
+    close(o.aux("dtv"), c["dtv"], f"{name}: dt v")
+    close(o.aux("dtP"), c["dtP"], f"{name}: dt P")
+    for key, exp in c["extra"].items():
+        close(o.aux(key), exp, f"{name}: {key}")
+    # the predictor and phi^{n+1} (getVStar, getVNew for phi, TTSH3D.cu:471-491,513)
+    close(o.aux("vstar"), c["v"] + DT * c["dtv"], f"{name}: v* = v + Dt dt v")
+    close(o.aux("pstar"), c["P"] + DT * c["dtP"], f"{name}: P* = P + Dt dt P")
+    close(o.get_state()[0], c["phi"] + DT * c["dtphi"], f"{name}: phi^(n+1) = phi + Dt dt phi")
+
+
+def _check(build):
+    def f(O):
+        check_case(O, build)
+    f.__name__ = "check_" + build.__name__[len("build_"):]
+    return f
+
+
+CHECKS = [_check(b) for b in BUILDERS]
 
 
 @pytest.mark.parametrize("check", CHECKS, ids=[c.__name__ for c in CHECKS])
