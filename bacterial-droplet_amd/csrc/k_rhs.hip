@@ -104,8 +104,8 @@ __device__ __forceinline__ Der4 der_rows(const Rows& r, int fo, const Coef& co)
 }
 
 
-template <int TX, int TY, int MINB>
-__global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ drops, const Work* __restrict__ work,
+template <int TX, int TY, int WPS>
+__global__ __launch_bounds__(TX * TY, WPS) void k_rhs(const Drop* __restrict__ drops, const Work* __restrict__ work,
                                                  const int* __restrict__ nwork_p, int parity, int kstep)
 {
     constexpr int NT = TX * TY;
@@ -120,10 +120,10 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
     // per position: phi, Px, Py, Pz x (f, dz, sz) | vx, vy, vz x (f, dz, sz) |
     // Sigma(9, [3a+b]), mu(3), mu0 of one plane
     constexpr int SP = 13, SV = 9, SS = 13;
-    // One-group tiles (MINB 1) are held to two waves per SIMD by LDS, so they
-    // can spend registers on unrolling the z-march by its 3-slot rotation,
-    // which removes the register moves of the rotation.
-    constexpr bool kUnroll3 = MINB == 1;
+    // Tiles of which LDS fits one work-group per CU (WPS 1: two waves per
+    // SIMD at most) can spend registers on unrolling the z-march by its
+    // 3-slot rotation, which removes the register moves of the rotation.
+    constexpr bool kUnroll3 = WPS == 1;
     __shared__ float zp[A2 * SP];
     __shared__ float zv[A1 * SV];
     __shared__ float sM[A1 * SS];
@@ -513,18 +513,19 @@ __global__ __launch_bounds__(TX * TY, MINB) void k_rhs(const Drop* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-template <int TX, int TY, int MINB>
+template <int TX, int TY, int WPS>
 static hipError_t launch_rhs_t(const Drop* drops, const Work* work, const int* nwork_p, int parity, int kstep,
                                int grid, hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    return launch_kernel(k_rhs<TX, TY, MINB>, dim3(grid), dim3(TX * TY), st, drops, work, nwork_p, parity, kstep);
+    return launch_kernel(k_rhs<TX, TY, WPS>, dim3(grid), dim3(TX * TY), st, drops, work, nwork_p, parity, kstep);
 }
 
-// Tile variants: TX x TY cells per work-group and the work-groups per CU the
-// register allocation targets.
+// Tile variants: TX x TY cells per work-group and the waves per SIMD the
+// register allocation targets (the second __launch_bounds__ argument: 2 allows
+// 256 VGPRs, 3 allows 168, 1 allows 512).
 struct RhsVar {
-    int tx, ty, minb;
+    int tx, ty, wps;
 };
 static const RhsVar kRhsVars[RHS_NVARIANTS] = {
     {32, 8, 2}, {16, 16, 2}, {64, 4, 2}, {16, 8, 2}, {32, 8, 3}, {32, 16, 1}, {64, 8, 1},

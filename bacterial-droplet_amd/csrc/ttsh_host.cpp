@@ -293,7 +293,7 @@ int ttsh_validate_params(const ttsh_params* p)
     if (p->Nb < 1) return fail(TTSH_EINVAL, "Nb must be >= 1");
     if (!(p->h > 0.0) || !(p->dt0 > 0.0)) return fail(TTSH_EINVAL, "h and dt0 must be positive");
     if (!(p->dte > 0.0)) return fail(TTSH_EINVAL, "dte must be positive");
-    if (p->rhs_variant < -1 || p->rhs_variant >= 7) return fail(TTSH_EINVAL, "rhs_variant out of range");
+    if (p->rhs_variant < -1 || p->rhs_variant >= kRhsVariants) return fail(TTSH_EINVAL, "rhs_variant out of range");
     if (p->rhs_tz < 0 || p->corr_tz < 0) return fail(TTSH_EINVAL, "tile depths must be >= 0");
     return TTSH_OK;
 }

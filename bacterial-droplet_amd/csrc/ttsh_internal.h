@@ -15,6 +15,8 @@ int write_conf_host(const char* path, int N, int expo_data, const float* phi, co
 int write_conf_bin_host(const char* path, int N, int m, double T, int64_t steps, const float* const fields[7]);
 // Level-contour radius R(theta) at 120 angles of an N x N [row][col] field.
 int contour_radius(int N, const float* field, double level, double* R, double* center);
+// Number of K_rhs tile variants (RhsVariant in ttsh_kernels.h).
+constexpr int kRhsVariants = 7;
 // Tuned K_rhs variant, K_rhs z-depth and projection z-depth for size N.
 bool tune_lookup(int N, int* rhs_variant, int* rhs_tz, int* proj_tz);
 // Tuned layout of a batch that fills the device: K_rhs variant, minimum

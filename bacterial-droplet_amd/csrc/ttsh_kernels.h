@@ -7,6 +7,7 @@
 #include <hip/hip_ext.h>
 
 #include "ttsh_dev.h"
+#include "ttsh_internal.h"
 
 namespace ttsh {
 
@@ -31,9 +32,11 @@ inline hipError_t launch_kernel(K kernel, dim3 grid, dim3 block, hipStream_t st,
 }
 
 // K_rhs tile variants (TX x TY cells per work-group, one cell per thread, and
-// the occupancy the register allocation targets); see kRhsVars in k_rhs.hip.
+// the waves per SIMD the register allocation targets); see kRhsVars in
+// k_rhs.hip.
 enum RhsVariant { RHS_32x8 = 0, RHS_16x16 = 1, RHS_64x4 = 2, RHS_16x8 = 3, RHS_32x8_O3 = 4, RHS_32x16 = 5,
                   RHS_64x8 = 6, RHS_NVARIANTS = 7 };
+static_assert(RHS_NVARIANTS == kRhsVariants, "tile variant count");
 
 // Tile of the divergence and corrector stencils.
 constexpr int kProjTX = 32;
