@@ -12,8 +12,9 @@
 //      registers, multiplies by W_N^{n1 k2} and stores to LDS;
 //   B: thread k2 < R2 loads the R1 values of its k2, does a length-R1 DFT in
 //      registers and writes X[k2 + R2 k1].
-// Register DFTs of length 2, 3, 4 are written out; longer ones (6, 8, 12,
-// 16) are composed from them with compile-time twiddles.  Sign convention:
+// Register DFTs of length 2, 3, 4, 5 are written out; longer ones (6, 8, 10,
+// 12, 15, 16, 20) are composed from them with compile-time twiddles.  The
+// plans cover the grid sizes of kGridSizes.  Sign convention:
 // forward e^{-2 pi i jk/N}, inverse e^{+2 pi i jk/N}, both unnormalised, as
 // cufftExecZ2Z.
 //
@@ -241,14 +242,25 @@ template <int DIR> struct Dft<20, DIR> : DftComp<4, 5, DIR> {};
 template <int N>
 struct Plan;
 template <> struct Plan<32>  { static constexpr int R1 = 4,  R2 = 8,  L = 32; };
+template <> struct Plan<36>  { static constexpr int R1 = 6,  R2 = 6,  L = 18; };
+template <> struct Plan<40>  { static constexpr int R1 = 8,  R2 = 5,  L = 20; };
 template <> struct Plan<48>  { static constexpr int R1 = 6,  R2 = 8,  L = 16; };
+template <> struct Plan<50>  { static constexpr int R1 = 10, R2 = 5,  L = 25; };
+template <> struct Plan<60>  { static constexpr int R1 = 12, R2 = 5,  L = 20; };
 template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  L = 32; };
+template <> struct Plan<72>  { static constexpr int R1 = 12, R2 = 6,  L = 18; };
 template <> struct Plan<80>  { static constexpr int R1 = 10, R2 = 8,  L = 16; };
+template <> struct Plan<90>  { static constexpr int R1 = 15, R2 = 6,  L = 15; };
 template <> struct Plan<96>  { static constexpr int R1 = 8,  R2 = 12, L = 16; };
+template <> struct Plan<100> { static constexpr int R1 = 10, R2 = 10, L = 25; };
+template <> struct Plan<120> { static constexpr int R1 = 12, R2 = 10, L = 20; };
 template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  L = 16; };
 template <> struct Plan<144> { static constexpr int R1 = 12, R2 = 12, L = 16; };
+template <> struct Plan<150> { static constexpr int R1 = 15, R2 = 10, L = 15; };
 template <> struct Plan<160> { static constexpr int R1 = 10, R2 = 16, L = 16; };
+template <> struct Plan<180> { static constexpr int R1 = 15, R2 = 12, L = 15; };
 template <> struct Plan<192> { static constexpr int R1 = 16, R2 = 12, L = 16; };
+template <> struct Plan<200> { static constexpr int R1 = 20, R2 = 10, L = 10; };
 template <> struct Plan<240> { static constexpr int R1 = 16, R2 = 15, L = 16; };
 template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, L = 16; };
 
@@ -550,19 +562,27 @@ static hipError_t launch_fft_xy_n(int mode, const Drop* drops, const Work* work,
 
 bool fft_xy_fused(int N) { return N <= kMaxFusedXy; }
 
+// f(std::integral_constant<int, N>) for the grid size N of the size list.
+template <int I = 0, class F>
+static hipError_t with_size(int N, F&& f)
+{
+    if constexpr (I == kNumGridSizes) {
+        return hipErrorInvalidValue;
+    } else {
+        if (N == kGridSizes[I]) return f(std::integral_constant<int, kGridSizes[I]>{});
+        return with_size<I + 1>(N, f);
+    }
+}
+
 hipError_t launch_fft_xy(int N, int mode, const Drop* drops, const Work* work, const int* nwork_p, int kstep, int grid,
                          hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    switch (N) {
-    case 32: return launch_fft_xy_n<32>(mode, drops, work, nwork_p, kstep, grid, st);
-    case 48: return launch_fft_xy_n<48>(mode, drops, work, nwork_p, kstep, grid, st);
-    case 64: return launch_fft_xy_n<64>(mode, drops, work, nwork_p, kstep, grid, st);
-    case 80: return launch_fft_xy_n<80>(mode, drops, work, nwork_p, kstep, grid, st);
-    case 96: return launch_fft_xy_n<96>(mode, drops, work, nwork_p, kstep, grid, st);
-    case 128: return launch_fft_xy_n<128>(mode, drops, work, nwork_p, kstep, grid, st);
-    default: return hipErrorInvalidValue;
-    }
+    return with_size(N, [&](auto n) {
+        constexpr int NS = decltype(n)::value;
+        if constexpr (NS <= kMaxFusedXy) return launch_fft_xy_n<NS>(mode, drops, work, nwork_p, kstep, grid, st);
+        else return hipErrorInvalidValue;
+    });
 }
 
 template <int N>
@@ -583,38 +603,19 @@ hipError_t launch_fft(int N, int axis, int mode, const Drop* drops, const Work* 
                       int grid, hipStream_t st)
 {
     if (grid <= 0) return hipSuccess;
-    switch (N) {
-    case 32: return launch_fft_n<32>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 48: return launch_fft_n<48>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 64: return launch_fft_n<64>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 80: return launch_fft_n<80>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 96: return launch_fft_n<96>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 128: return launch_fft_n<128>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 144: return launch_fft_n<144>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 160: return launch_fft_n<160>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 192: return launch_fft_n<192>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 240: return launch_fft_n<240>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    case 256: return launch_fft_n<256>(axis, mode, drops, work, nwork_p, kstep, grid, st);
-    default: return hipErrorInvalidValue;
-    }
+    return with_size(N, [&](auto n) {
+        return launch_fft_n<decltype(n)::value>(axis, mode, drops, work, nwork_p, kstep, grid, st);
+    });
 }
 
 int fft_lines_per_group(int N)
 {
-    switch (N) {
-    case 32: return PlanT<32>::L;
-    case 48: return PlanT<48>::L;
-    case 64: return PlanT<64>::L;
-    case 80: return PlanT<80>::L;
-    case 96: return PlanT<96>::L;
-    case 128: return PlanT<128>::L;
-    case 144: return PlanT<144>::L;
-    case 160: return PlanT<160>::L;
-    case 192: return PlanT<192>::L;
-    case 240: return PlanT<240>::L;
-    case 256: return PlanT<256>::L;
-    default: return 0;
-    }
+    int L = 0;
+    (void)with_size(N, [&](auto n) {
+        L = PlanT<decltype(n)::value>::L;
+        return hipSuccess;
+    });
+    return L;
 }
 
 }  // namespace ttsh

@@ -56,15 +56,15 @@ static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_xy", "K_ff
 // sums), and per grid size the x/y-pass table, the z-pass table and the
 // z-plane table of the fused x+y pass.
 enum { T_RHS = 0, T_PROJ = 1, T_STAT = 2, T_CORR = 3, T_FFT0 = 4 };
-constexpr int kNTables = T_FFT0 + 3 * kNumSizes;
+constexpr int kNTables = T_FFT0 + 3 * kNumGridSizes;
 static inline int t_xy(int s) { return T_FFT0 + 3 * s; }      // line groups of x or y passes
 static inline int t_z(int s) { return T_FFT0 + 3 * s + 1; }   // line groups of the z pass
 static inline int t_pl(int s) { return T_FFT0 + 3 * s + 2; }  // z-planes of the fused x+y pass
 
 static int size_index(int N)
 {
-    for (int s = 0; s < kNumSizes; ++s)
-        if (kSizes[s] == N) return s;
+    for (int s = 0; s < kNumGridSizes; ++s)
+        if (kGridSizes[s] == N) return s;
     return -1;
 }
 
@@ -773,7 +773,7 @@ struct Engine {
         // batch), otherwise the per-plane transform work leaves CUs idle and
         // the single-axis passes are faster.
         xy_mask = 0;
-        for (int sz = 0; sz < kNumSizes; ++sz)
+        for (int sz = 0; sz < kNumGridSizes; ++sz)
             if (tab[t_pl(sz)].n >= 2 * cus) xy_mask |= 1u << sz;
         HIPCHK(hipMemcpyAsync(d_counts, h_counts, sizeof(int) * kNTables, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_drops, h_drops.data(), sizeof(Drop) * slots.size(), hipMemcpyHostToDevice, st));
@@ -786,7 +786,7 @@ struct Engine {
     unsigned sizes_present() const
     {
         unsigned m = 0;
-        for (int s = 0; s < kNumSizes; ++s)
+        for (int s = 0; s < kNumGridSizes; ++s)
             if (tab[t_z(s)].n > 0) m |= 1u << s;
         return m;
     }
@@ -807,7 +807,7 @@ struct Engine {
             int kind, what, s, t;
         };
         enum { W_RHS, W_DIV, W_STAT, W_FFT, W_FFT_XY, W_CORR };
-        L list[4 + 5 * kNumSizes];
+        L list[4 + 5 * kNumGridSizes];
         int nl = 0;
         auto add = [&](int kind, int what, int s, int t) {
             if (grid(t) > 0) list[nl++] = L{kind, what, s, t};
@@ -820,7 +820,7 @@ struct Engine {
         const int axes[5] = {0, 1, 2, 1, 0};
         const int modes[5] = {FFT_MODE_FWD, FFT_MODE_FWD, FFT_MODE_POISSON, FFT_MODE_INV, FFT_MODE_INV};
         for (int f = 0; f < 5; ++f)
-            for (int s = 0; s < kNumSizes; ++s) {
+            for (int s = 0; s < kNumGridSizes; ++s) {
                 if (!on(s)) continue;
                 const bool fused = (((graph ? gxy_mask : xy_mask) >> s) & 1u) != 0;
                 if (axes[f] != 2 && fused) {
@@ -839,7 +839,7 @@ struct Engine {
                 g_launch_ev.stop = (i == nl - 1 || list[i + 1].kind != l.kind) ? ev[2 * l.kind + 1] : nullptr;
                 if (launched) launched[l.kind] = true;
             }
-            const int N = kSizes[l.s];
+            const int N = kGridSizes[l.s];
             const int f = l.kind - KK_FX;
             hipError_t e = hipSuccess;
             switch (l.what) {

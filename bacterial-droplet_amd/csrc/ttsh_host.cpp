@@ -42,16 +42,14 @@ int fft_plan(int N, int* rad, int maxst)
     return (m == 1) ? n : 0;
 }
 
-// The grid sizes with a compile-time FFT plan (k_fft.hip).
-static const int kPlanSizes[] = {32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256};
 
 bool supported_N(int N, std::string* why)
 {
-    for (int s : kPlanSizes)
+    for (int s : kGridSizes)
         if (s == N) return true;
     if (why) {
         std::string list;
-        for (int s : kPlanSizes) list += (list.empty() ? "" : ", ") + std::to_string(s);
+        for (int s : kGridSizes) list += (list.empty() ? "" : ", ") + std::to_string(s);
         *why = "no FFT plan for N = " + std::to_string(N) + " (sizes: " + list + ")";
     }
     return false;

@@ -42,10 +42,6 @@ static_assert(RHS_NVARIANTS == kRhsVariants, "tile variant count");
 constexpr int kProjTX = 32;
 constexpr int kProjTY = 8;
 
-// Grid sizes with a compile-time FFT plan (prime factors 2, 3 and 5).
-constexpr int kNumSizes = 11;
-constexpr int kSizes[kNumSizes] = {32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256};
-
 // FFT pass kinds (k_fft.hip).
 enum { FFT_MODE_FWD = 0, FFT_MODE_INV = 1, FFT_MODE_POISSON = 2 };
 

@@ -169,8 +169,10 @@ def main(argv=None):
         entries[N] = best
         allres[N] = res
         print(f"N={N}: best rhs variant {best[0]} tz {best[1]}, proj tz {best[2]}", flush=True)
-    _, batch = read_table(a.out)
-    write_table(entries, a.out, batch)
+    # sizes not tuned in this run keep their entries
+    kept, batch = read_table(a.out)
+    kept.update(entries)
+    write_table(kept, a.out, batch)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(allres, f, indent=1)

@@ -17,12 +17,13 @@
  *   - A ttsh_sim or ttsh_batch owns its device memory and one HIP stream.
  *     Handles are not thread-safe; use one handle per thread.
  *   - Grids are cubic (Nx = Ny = Nz = N, as the reference's cuFFT plan
- *     requires, TTSH3D.cu:1123) with N in {32, 48, 64, 80, 96, 128, 144,
- *     160, 192, 240, 256}: the sizes with a compile-time FFT plan (prime
- *     factors 2, 3 and 5).  The reference's default and swept N is 64
+ *     requires, TTSH3D.cu:1123) with N in {32, 36, 40, 48, 50, 60, 64, 72,
+ *     80, 90, 96, 100, 120, 128, 144, 150, 160, 180, 192, 200, 240, 256}:
+ *     the sizes with a compile-time FFT plan (even, prime factors 2, 3 and
+ *     5, no factor 27 or 125).  The reference's default and swept N is 64
  *     (run_cuda.pl:14-16, forEach.pl:30); any other N its cuFFT plan accepts
- *     -- below 32, above 256, or not in that list (e.g. 16, 40, 100, 200) --
- *     is refused here with TTSH_EUNSUPPORTED.
+ *     -- below 32, above 256, odd, or with a factor 7, 27 or 125 (e.g. 16, 54,
+ *     70, 250) -- is refused here with TTSH_EUNSUPPORTED.
  */
 #ifndef TTSH3D_H
 #define TTSH3D_H

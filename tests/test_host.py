@@ -104,8 +104,10 @@ def test_short_or_missing_input(ttsh, tmp_path):
 @pytest.mark.parametrize("kw,code", [
     (dict(TimeScheme=3), "EUNSUPPORTED"),
     (dict(Nx=64, Ny=64, Nz=32), "EUNSUPPORTED"),
-    (dict(N=100), "EUNSUPPORTED"),
-    (dict(N=40), "EUNSUPPORTED"),
+    (dict(N=54), "EUNSUPPORTED"),
+    (dict(N=70), "EUNSUPPORTED"),
+    (dict(N=45), "EUNSUPPORTED"),
+    (dict(N=250), "EUNSUPPORTED"),
     (dict(N=16), "EUNSUPPORTED"),
     (dict(N=512), "EUNSUPPORTED"),
     (dict(Nb=0), "EINVAL"),
@@ -282,8 +284,32 @@ def test_tuning_table_batch_line(ttsh, tmp_path):
         ttsh.tuning_reload(None)
 
 
-@pytest.mark.parametrize("N", [32, 48, 64, 80, 96, 128, 144, 160, 192, 240, 256])
+PLANNED = [32, 36, 40, 48, 50, 60, 64, 72, 80, 90, 96, 100, 120, 128, 144, 150, 160, 180, 192, 200, 240, 256]
+
+
+def test_planned_sizes_are_the_even_smooth_sizes_without_27_or_125():
+    """Every even N in [32, 256] with prime factors 2, 3, 5 splits as R1 x R2
+    over the register DFT lengths {2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20}
+    unless 27 or 125 divides it; those are the planned sizes."""
+    def smooth(n):
+        for q in (2, 3, 5):
+            while n % q == 0:
+                n //= q
+        return n == 1
+    lengths = {2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20}
+    even = [n for n in range(32, 257, 2) if smooth(n)]
+    assert [n for n in even if n % 27 and n % 125] == PLANNED
+    assert [n for n in even if any(n % a == 0 and n // a in lengths for a in lengths)] == PLANNED
+
+
+@pytest.mark.parametrize("N", [54, 108, 162, 216, 250])
+def test_sizes_without_a_plan_are_refused(ttsh, N):
+    p = ttsh.default_params(N=N)
+    assert ttsh.lib().ttsh_validate_params(ctypes.byref(p)) == ttsh.EUNSUPPORTED
+
+
+@pytest.mark.parametrize("N", PLANNED)
 def test_every_planned_size_validates(ttsh, N):
-    """The sizes with a compile-time FFT plan (prime factors 2, 3 and 5)."""
+    """The sizes with a compile-time FFT plan."""
     p = ttsh.default_params(N=N)
     assert ttsh.lib().ttsh_validate_params(ctypes.byref(p)) == 0
