@@ -25,8 +25,7 @@ import os
 import sys
 import time
 
-from . import (F64, TTSHError, Batch, Simulation, default_params, init_droplet, lib, LIB_PATH, tuning_lookup,
-               tuning_lookup_batch, tuning_reload)
+from . import F64, TTSHError, Batch, Simulation, default_params, init_droplet, lib, LIB_PATH, tuning_reload
 
 RHS_VARIANTS = {0: (32, 8), 1: (16, 16), 2: (64, 4), 3: (16, 8), 4: (32, 8), 5: (32, 16), 6: (64, 8)}
 DEFAULT_TABLE = os.path.join(os.path.dirname(LIB_PATH), "tuned.txt")
@@ -105,7 +104,6 @@ def tune_batch(table: str, steps: int = 48, log=print):
     (its per-size lines kept) and re-read; returns ((variant, rhs_tz,
     corr_tz), ms per batch step, all results)."""
     from . import workloads
-    import math
     entries, _ = read_table(table)
     cases = workloads.sweep_mix(__import__("ttsh3d"))
     sizes = sorted({L for _, L, _ in cases})

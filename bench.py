@@ -40,7 +40,6 @@ sims/hour and p50/p99 completion latency of that bounded sample.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
