@@ -10,9 +10,10 @@ cA = -2, random-unit P) is `value`; the 256^3 droplet (R = 80, xi = 3, noisy
 aster P), whose 1.55 GB working set exceeds the 256 MB Infinity Cache, is
 reported beside it.  A "step" is one explicit-Euler step of the reference's
 EEuler (TTSH3D.cu:347-359) on the droplet, device-resident.  W warm-up
-steps, then K steps through ttsh_profile, whose kernels record HIP events at
-their own start and end on the step's stream (kernel_ms: each kind's average
-duration, as a kernel trace measures it), then exactly K timed steps between
+steps, then max(K, 100) steps through ttsh_profile, whose kernels record HIP
+events at their own start and end on the step's stream (kernel_ms: each
+kind's average duration, as a kernel trace measures it; these steps also
+bring the GPU to its sustained clock), then exactly K timed steps between
 two synchronisations, launched as a caller launches them (ttsh_step: the
 step's kernels queued back to back on one stream); value = N^3 * K / wall
 seconds / 1e6.
@@ -114,10 +115,14 @@ def single_leg(ttsh, N, steps, warmup):
     sim.set_state(phi, P, v)
     sim.step(warmup)
     sim.synchronize()
-    # per-kernel durations: K steps whose kernels record HIP events at their
-    # own start and end on the step's stream (plain launches)
+    # per-kernel durations: max(K, 100) steps whose kernels record HIP events
+    # at their own start and end on the step's stream (plain launches).  They
+    # also keep the GPU busy long enough (~20 ms at 128^3) to reach its
+    # sustained clock before the timed region: after an idle spell the first
+    # ~15 ms of steps run up to 7% slower, which a few warm-up steps do not cover.
+    nprof = max(steps, 100)
     t1 = time.perf_counter()
-    kms = sim.profile(steps)
+    kms = sim.profile(nprof)
     sim.synchronize()
     wall_prof = time.perf_counter() - t1
     # timed region: K steps as a caller runs them (ttsh_step, no
@@ -130,14 +135,14 @@ def single_leg(ttsh, N, steps, warmup):
     sim.close()
     if not finite:
         raise RuntimeError(f"non-finite state after the timed steps at {N}^3: {red}")
-    avg = {k: t / steps for k, t in kms.items()}
+    avg = {k: t / nprof for k, t in kms.items()}
     traffic = None
     tpath = os.path.join(ROOT, "bench_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(str(N))
     value = cells * steps / wall / 1e6
-    return {"value": value, "ms_per_step": wall * 1e3 / steps, "ms_per_step_instrumented": wall_prof * 1e3 / steps,
+    return {"value": value, "ms_per_step": wall * 1e3 / steps, "ms_per_step_instrumented": wall_prof * 1e3 / nprof,
             "kernels": kernel_table(avg, N, traffic), "avg": avg, "traffic": traffic,
             "reductions_fp64": {"mass": red[0], "mean_abs_P": red[1], "kinetic_energy": red[2]}, "params": p,
             "state": (phi, P, v)}
