@@ -332,8 +332,7 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
         const Work wk = work[w];
         const Drop& D = drops[wk.d];
         if (kstep >= D.nsteps) continue;
-        __syncthreads();
-        for (int i = tid; i < N; i += blockDim.x) stw[i] = ld2(glb(D.tw), i);
+        __syncthreads();  // the previous item has read sx and stw
         // first element of line l
         size_t base;
         int lx = 0, ly = 0;  // for the Poisson factor: the line's fixed indices
@@ -357,19 +356,26 @@ __global__ __launch_bounds__(PlanT<N>::NB) void k_fft(const Drop* __restrict__ d
         // power of two: exact)
         float isg = 1.f;
         if constexpr (AXIS == 0 && MODE == FFT_FWD) isg = 1.f / glb(D.sigma)[0];
-        __syncthreads();  // stw
 
         constexpr int DIR0 = MODE == FFT_INV ? 1 : -1;
         float2 u[R1 > R2 ? R1 : R2];
         // ---- step A: R1 threads, length-R2 DFTs -----------------------------
+        // The line's loads are issued before the twiddle table's, so both
+        // are in flight together; the table's LDS stores wait for all of them.
         if (active && t < R1) {
 #pragma unroll
             for (int n2 = 0; n2 < R2; ++n2) u[n2] = ld2(c, base + (size_t)(t + R1 * n2) * es);
+        }
+        for (int i = tid; i < N; i += blockDim.x) stw[i] = ld2(glb(D.tw), i);
+        if (active && t < R1) {
             if constexpr (AXIS == 0 && MODE == FFT_FWD) {
 #pragma unroll
                 for (int n2 = 0; n2 < R2; ++n2) u[n2].y *= isg;
             }
             Dft<R2, DIR0>::run(u);
+        }
+        __syncthreads();  // stw
+        if (active && t < R1) {
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) {
                 const float2 wv = stw[(t * k2) % N];
