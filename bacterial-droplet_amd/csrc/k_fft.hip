@@ -521,19 +521,31 @@ __global__ __launch_bounds__(XyCfg<N>::NT) void k_fft_xy(const Drop* __restrict_
         if (kstep >= D.nsteps) continue;
         const gptr<fvec4> cp = (gptr<fvec4>)(glb(D.c) + (size_t)wk.b * N * N);
         __syncthreads();  // the previous item's stores have read the plane
+        // load: two complex elements of one row per 16-byte access, all of
+        // the thread's accesses issued before the twiddle table's and sigma's
+        // so that they are in flight together; the forward pass divides the
+        // imaginary part by sigma (a power of two)
+        constexpr int NQ = (N * N / 2 + NT - 1) / NT;
+        fvec4 q[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int i = tid + k * NT;
+            if (i < N * N / 2) q[k] = cp[i];
+        }
         for (int i = tid; i < N; i += NT) stw[i] = ld2(glb(D.tw), i);
-        // load: two complex elements of one row per 16-byte access; the
-        // forward pass divides the imaginary part by sigma (a power of two)
         const float isg = MODE == FFT_FWD ? 1.f / glb(D.sigma)[0] : 1.f;
-        for (int i = tid; i < N * N / 2; i += NT) {
-            fvec4 q = cp[i];
-            if constexpr (MODE == FFT_FWD) {
-                q.y *= isg;
-                q.w *= isg;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const int i = tid + k * NT;
+            if (i < N * N / 2) {
+                if constexpr (MODE == FFT_FWD) {
+                    q[k].y *= isg;
+                    q[k].w *= isg;
+                }
+                const int y = (2 * i) / N, x = (2 * i) % N;
+                pl[y * P + x] = f2(q[k].x, q[k].y);
+                pl[y * P + x + 1] = f2(q[k].z, q[k].w);
             }
-            const int y = (2 * i) / N, x = (2 * i) % N;
-            pl[y * P + x] = f2(q.x, q.y);
-            pl[y * P + x + 1] = f2(q.z, q.w);
         }
         __syncthreads();
         if constexpr (MODE == FFT_INV) {
