@@ -302,6 +302,22 @@ def test_planned_sizes_are_the_even_smooth_sizes_without_27_or_125():
     assert [n for n in even if any(n % a == 0 and n // a in lengths for a in lengths)] == PLANNED
 
 
+def test_shipped_tuning_table_covers_every_planned_size(ttsh):
+    """tuned.txt beside the library holds an entry for every planned size, each
+    with a K_rhs tile that fits the size, and the batch line; the library
+    reads the same entries back."""
+    from ttsh3d import autotune
+    entries, batch = autotune.read_table(autotune.DEFAULT_TABLE)
+    assert sorted(entries) == PLANNED
+    assert batch is not None
+    for N, (v, rtz, ptz) in entries.items():
+        tx, ty = autotune.RHS_VARIANTS[v]
+        assert N >= tx + 2 and N >= ty + 2 and 0 < rtz <= N and 0 < ptz <= N, (N, v, rtz, ptz)
+    ttsh.tuning_reload(None)
+    for N in (36, 128, 256):
+        assert ttsh.tuning_lookup(N) == entries[N]
+
+
 @pytest.mark.parametrize("N", [54, 108, 162, 216, 250])
 def test_sizes_without_a_plan_are_refused(ttsh, N):
     p = ttsh.default_params(N=N)
