@@ -57,6 +57,7 @@ static const char* const kKindNames[KK_N] = {"K_rhs", "K_div", "K_fft_xy", "K_ff
 // z-plane table of the fused x+y pass.
 enum { T_RHS = 0, T_PROJ = 1, T_STAT = 2, T_CORR = 3, T_FFT0 = 4 };
 constexpr int kNTables = T_FFT0 + 3 * kNumGridSizes;
+static_assert(kNumGridSizes <= 32, "sizes are kept as bits of a 32-bit mask");
 static inline int t_xy(int s) { return T_FFT0 + 3 * s; }      // line groups of x or y passes
 static inline int t_z(int s) { return T_FFT0 + 3 * s + 1; }   // line groups of the z pass
 static inline int t_pl(int s) { return T_FFT0 + 3 * s + 2; }  // z-planes of the fused x+y pass

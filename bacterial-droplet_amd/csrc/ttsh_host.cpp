@@ -42,7 +42,6 @@ int fft_plan(int N, int* rad, int maxst)
     return (m == 1) ? n : 0;
 }
 
-
 bool supported_N(int N, std::string* why)
 {
     for (int s : kGridSizes)
